@@ -1,0 +1,406 @@
+"""Pins the CPU oracle against the reference's golden data and closed forms.
+
+What pins each oracle function (C# captures unless stated):
+
+* pyramid            byte-exact vs 29 captured pyramids + the 7x5 invalid-value
+                     fixture; independent numpy max reduction
+* build_samples      captured ray samples (1e-10), 29 ray-fit passes
+* fit_segment        captured 18-field segments (6e-7, the Numba-path bound)
+* segments           captured 16-azimuth x 16-center x 2-DEM subpatch tensor
+                     (1e-5 + path error 1e-5, the Numba-path bounds); the polar
+                     geometry of the synthetic configs is pinned by the round
+                     trips and the path-vs-samples invariant below
+* march_ray          captured C# traces (actions, levels and cells exact) in the
+                     float32 + cell-exit-cap mode the captures encode; float64
+                     method mode within 1e-6 of those maxima; flat sphere closed form
+* horizon_tile       captured one-pixel x 1440-azimuth x 2-pass production buffer
+* sun_fraction       captured BuilderSunFraction bytes (byte-exact), 255/127/0
+* psr_patch          captured 3 x 128x128 PSR outputs (byte-exact), strict limb
+* psr_reduce         keeps the highest elevation per viewpoint bin
+* cbin codec         round trip, quantization bound, structural rules
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from tests import fixtures as fx
+from lunarscout_amd.geometry import DemGrid, ProjectionParameters, local_direction_vector
+
+
+def _case_ids():
+    return [c["id"] for c in fx.rays_metadata()["cases"]]
+
+
+def _numpy_max_reduction(source: np.ndarray) -> list[np.ndarray]:
+    levels = []
+    current = source
+    while current.shape != (1, 1):
+        h, w = current.shape
+        out = np.full(((h + 3) // 4, (w + 3) // 4), -32000.0, dtype=np.float32)
+        for r in range(out.shape[0]):
+            for c in range(out.shape[1]):
+                block = current[r * 4:(r + 1) * 4, c * 4:(c + 1) * 4]
+                valid = block[np.isfinite(block) & (block > -20000.0)]
+                if valid.size:
+                    out[r, c] = valid.max()
+        levels.append(out)
+        current = out
+    return levels
+
+
+def test_pyramids_match_every_captured_pyramid_byte_exact():
+    arrays = fx.rays_arrays()
+    prefixes = sorted({n.split("__pyramid__")[0] for n in arrays if "__pyramid__level_metadata" in n})
+    assert len(prefixes) >= 29
+    for prefix in prefixes:
+        if prefix.startswith("invalid_values"):
+            continue
+        expected_levels, expected_mips = fx.pyramid_levels(prefix)
+        levels, mips = oracle.build_pyramid(arrays[f"{prefix}__elevation_m"])
+        np.testing.assert_array_equal(levels, expected_levels, err_msg=prefix)
+        assert mips.tobytes() == expected_mips.tobytes(), prefix
+
+
+def test_invalid_value_pyramid_fixture_and_independent_reduction():
+    a = fx.rays_arrays()
+    base = "invalid_values_odd_dimensions__pyramid_fixture"
+    elevation = a[f"{base}__elevation_m"]
+    levels, mips = oracle.build_pyramid(elevation)
+    np.testing.assert_array_equal(levels, a[f"{base}__level_metadata"].astype(np.int32))
+    expected = np.concatenate([a[f"{base}__level_1"].ravel(), a[f"{base}__level_2"].ravel()])
+    assert mips.tobytes() == expected.astype(np.float32).tobytes()
+    rng = np.random.default_rng(3)
+    dem = rng.normal(0, 100, (37, 53)).astype(np.float32)
+    dem[rng.random(dem.shape) < 0.1] = np.nan
+    dem[rng.random(dem.shape) < 0.05] = -30000.0
+    _, mips = oracle.build_pyramid(dem)
+    np.testing.assert_array_equal(mips, np.concatenate([l.ravel() for l in _numpy_max_reduction(dem)]))
+
+
+def _fit_jobs():
+    meta = fx.rays_metadata()
+    arrays = fx.rays_arrays()
+    for case in meta["cases"]:
+        dems = [fx.dem(f"{case['id']}__dem_{d['index']}") for d in case["dems"]]
+        obs = case["observer"]
+        primary = dems[0]
+        # DemGrid.elevation at integer pixels (geometry.py:104-116)
+        terrain = float(primary.elevation_m[obs["pixel_y"], obs["pixel_x"]])
+        radius = primary.projection.radius_m + terrain
+        for fit_index, fit in enumerate(case["ray_fit_passes"]):
+            prefix = f"{case['id']}__ray_fit_pass_{fit_index}"
+            yield case, dems, fit, prefix, radius, arrays
+
+
+def test_ray_samples_and_segments_match_csharp():
+    worst_samples = worst_segments = worst_path = 0.0
+    jobs = 0
+    for case, dems, fit, prefix, radius, arrays in _fit_jobs():
+        observer = arrays[f"{prefix}__observer_vector_moon_centered_m"]
+        direction = arrays[f"{prefix}__nominal_direction_moon_centered"]
+        actual = oracle.build_samples(dems[fit["dem_index"]], observer, direction,
+                                      fit["requested_start_distance_m"], fit["ray_limit_m"])
+        expected = fx.samples(prefix)
+        assert actual.shape == expected.shape, prefix
+        worst_samples = max(worst_samples, float(np.max(np.abs(actual - expected))))
+        segment = oracle.fit_segment(actual, fit["map_resolution_m"], observer, radius,
+                                     fit["requested_start_distance_m"])
+        expected_segment = arrays[f"{prefix}__segment_values"]
+        worst_segments = max(worst_segments, float(np.max(np.abs(segment - expected_segment))))
+        # fitted path stays within 0.05 px of the samples (a closed-form invariant)
+        d = actual[:, 0] / 1000.0 - segment[12]
+        x = segment[2] + sum(segment[3 + k] * d ** k for k in range(1, 5))
+        y = segment[3] + sum(segment[7 + k] * d ** k for k in range(1, 5))
+        worst_path = max(worst_path, float(np.max(np.abs(x - actual[:, 1]))),
+                         float(np.max(np.abs(y - actual[:, 2]))))
+        jobs += 1
+    assert jobs >= 29
+    assert worst_samples < 1e-10
+    assert worst_segments < 6e-7
+    assert worst_path < 0.05
+
+
+def test_subpatch_segment_tensor_matches_csharp():
+    meta = fx.rays_metadata()
+    arrays = fx.rays_arrays()
+    fixture = next(f for f in meta["subpatch_fixtures"] if f["id"] == "boundary_halo_multi_dem_16az")
+    cfg = fixture["configuration"]
+    dems = [fx.dem("multi_dem_outer_obstacle_east__dem_0"),
+            fx.dem("multi_dem_outer_obstacle_east__dem_1")]
+    actual = oracle.segments(
+        dems, tile_column=cfg["tile_column"], tile_row=cfg["tile_row"],
+        tile_width=cfg["tile_width"], azimuth_count=cfg["azimuth_count"],
+        maximum_distance_m=cfg["max_distance_m"],
+        observer_elevation_m=cfg["observer_elevation_m"], subpatch_size=cfg["subpatch_size"])
+    expected = arrays[f"{fixture['id']}__subpatch_fixture__segment_values"]
+    np.testing.assert_allclose(actual, expected, rtol=0, atol=1e-5)
+    worst = 0.0
+    for seg, ref in zip(actual.reshape(-1, 18), expected.reshape(-1, 18)):
+        dist = np.linspace(max(seg[12], ref[12]), min(seg[13], ref[13]), 65)
+        def path(s):
+            d = dist - s[12]
+            return (s[2] + sum(s[3 + k] * d ** k for k in range(1, 5)),
+                    s[3] + sum(s[7 + k] * d ** k for k in range(1, 5)))
+        (ax, ay), (ex, ey) = path(seg.astype(np.float64)), path(ref.astype(np.float64))
+        worst = max(worst, float(np.max(np.hypot(ax - ex, ay - ey))))
+    assert worst < 1e-5
+
+
+def _assert_trace(actual, expected, *, elevation_atol, slope_atol):
+    assert actual.shape == expected.shape
+    np.testing.assert_array_equal(actual[:, 2:5], expected[:, 2:5])
+    np.testing.assert_array_equal(actual[:, 11], expected[:, 11])
+    np.testing.assert_allclose(actual[:, 0], expected[:, 0], rtol=0, atol=4e-6)
+    np.testing.assert_allclose(actual[:, 1], expected[:, 1], rtol=0, atol=6e-3)
+    np.testing.assert_allclose(actual[:, 5:7], expected[:, 5:7], rtol=0, atol=2e-4)
+    np.testing.assert_allclose(actual[:, 7], expected[:, 7], rtol=0, atol=2e-8)
+    np.testing.assert_allclose(actual[:, 8], expected[:, 8], rtol=0, atol=elevation_atol,
+                               equal_nan=True)
+    np.testing.assert_allclose(actual[:, 9], expected[:, 9], rtol=0, atol=slope_atol,
+                               equal_nan=True)
+    np.testing.assert_allclose(actual[:, 10], expected[:, 10], rtol=0, atol=6e-6)
+
+
+SHIFTS = ((4, 4), (-4, 4), (4, -4), (-4, -4))
+
+
+def _production_case():
+    base = "single_pixel_multi_dem_production__horizon_buffer_fixture"
+    capture = fx.phase4d()
+    segment = fx.interpolate_segments(np.asarray(capture["segments"], np.float32), SHIFTS,
+                                      1.0, 0.5, 0.5)
+    levels, mips = fx.pyramid_levels(f"{base}__dem_1")
+    pyramid = oracle.OraclePyramid(fx.dem(f"{base}__dem_1"), levels, mips)
+    primary = fx.dem(f"{base}__dem_0")
+    observer_z = float(primary.elevation_m[20, 20])
+    return capture["production_hierarchy_case"], segment, pyramid, observer_z
+
+
+def _boundary_case():
+    capture = fx.phase4d()["bilinear_boundary_case"]
+    case = next(c for c in fx.rays_metadata()["cases"] if c["id"] == "single_obstacle_north")
+    dem = fx.dem("single_obstacle_north__dem_0")
+    segment = fx.interpolate_segments(np.asarray(capture["segments"], np.float32), SHIFTS,
+                                      1.0, 0.5, 0.5)
+    obs = case["observer"]
+    observer_z = float(dem.elevation_m[obs["pixel_y"], obs["pixel_x"]]) + obs["elevation_m"]
+    pyramid = oracle.OraclePyramid(dem)
+    resolution = case["ray_fit_passes"][0]["map_resolution_m"]
+    return capture, segment, pyramid, observer_z, dem.projection.radius_m, resolution
+
+
+def test_faithful_capped_march_reproduces_csharp_production_trace():
+    capture, segment, pyramid, observer_z = _production_case()
+    slope, trace = oracle.march_ray(
+        segment, pyramid, observer_z=observer_z, radius=1737400.0, map_resolution=30.0,
+        pass_index=1, mode=oracle.FLOAT32 | oracle.CAP | oracle.REFERENCE_SLOPE,
+        trace_capacity=4096)
+    _assert_trace(trace, fx.trace_from_capture(capture), elevation_atol=0.5, slope_atol=2e-5)
+    assert slope == pytest.approx(capture["csharp_hierarchy_maximum_slope"], abs=3e-8)
+    assert np.count_nonzero(trace[:, 11] == 1) > 0  # blocks were culled
+
+
+def test_faithful_capped_march_reproduces_csharp_boundary_trace():
+    capture, segment, pyramid, observer_z, radius, res = _boundary_case()
+    slope, trace = oracle.march_ray(
+        segment, pyramid, observer_z=observer_z, radius=radius, map_resolution=res,
+        pass_index=0, mode=oracle.FLOAT32 | oracle.CAP | oracle.REFERENCE_SLOPE,
+        trace_capacity=4096)
+    _assert_trace(trace, fx.trace_from_capture(capture), elevation_atol=2e-3, slope_atol=2e-6)
+    assert slope == pytest.approx(capture["csharp_hierarchy_maximum_slope"], abs=1e-6)
+
+
+def test_method_mode_agrees_with_captured_maxima():
+    """Float64 evaluation of the capped schedule lands on the captured maxima.
+
+    The captured value carries the float32 radius quantization of the reference
+    slope form (R + h rounds to 0.125 m; at 29 km that is ~4e-6 in slope), so
+    float64 is held to 1e-5 and the stable float32 form to 1e-5 as well.
+    """
+    capture, segment, pyramid, observer_z = _production_case()
+    for mode in (oracle.CAP, oracle.FLOAT32 | oracle.CAP):
+        slope, _ = oracle.march_ray(segment, pyramid, observer_z=observer_z, radius=1737400.0,
+                                    map_resolution=30.0, pass_index=1, mode=mode)
+        assert slope == pytest.approx(capture["csharp_hierarchy_maximum_slope"], abs=1e-5)
+    capture, segment, pyramid, observer_z, radius, res = _boundary_case()
+    slope, _ = oracle.march_ray(segment, pyramid, observer_z=observer_z, radius=radius,
+                                map_resolution=res, pass_index=0, mode=oracle.CAP)
+    assert slope == pytest.approx(capture["csharp_hierarchy_maximum_slope"], abs=1e-5)
+
+
+def test_uncapped_schedule_differs_from_capped_only_where_the_cap_binds():
+    """The production schedule drops exactly the cell-exit cap (cuda_backend.py:862-866).
+
+    Both traces agree row for row until the first level-0 advance that the cap
+    shortened; there the capped advance is the cell exit + 1e-6 km and the
+    uncapped advance is longer.
+    """
+    for case in (_production_case, _boundary_case):
+        args = case()
+        if case is _production_case:
+            _, segment, pyramid, observer_z = args
+            radius, res, pass_index = 1737400.0, 30.0, 1
+        else:
+            _, segment, pyramid, observer_z, radius, res = args
+            pass_index = 0
+        kw = dict(observer_z=observer_z, radius=radius, map_resolution=res,
+                  pass_index=pass_index, trace_capacity=8192)
+        _, capped = oracle.march_ray(segment, pyramid, mode=oracle.CAP, **kw)
+        _, free = oracle.march_ray(segment, pyramid, mode=0, **kw)
+        n = min(len(capped), len(free))
+        same = (capped[:n] == free[:n]) | (np.isnan(capped[:n]) & np.isnan(free[:n]))
+        differs = np.flatnonzero(~same.all(axis=1))
+        assert len(differs) > 0
+        first = differs[0]
+        assert capped[first, 11] == 4 and free[first, 11] == 4
+        np.testing.assert_array_equal(capped[first, :10], free[first, :10])
+        assert free[first, 10] > capped[first, 10]
+
+
+def test_production_buffer_final_slopes_and_degrees():
+    """One pixel x 1440 azimuths x 2 DEM passes of the captured production buffer.
+
+    Azimuth 360 marches straight onto a single-pixel 5000 m spike; there the
+    captured buffer (0.16534) and the captured diagnostic trace of the same ray
+    (0.16593, reproduced exactly above) disagree with each other, so that bin is
+    held to the looser bound only.
+    """
+    arrays = fx.rays_arrays()
+    base = "single_pixel_multi_dem_production__horizon_buffer_fixture"
+    dems = [fx.dem(f"{base}__dem_{i}") for i in range(2)]
+    pyramids = [oracle.OraclePyramid(d, *fx.pyramid_levels(f"{base}__dem_{i}"))
+                for i, d in enumerate(dems)]
+    segs = oracle.segments(dems, tile_column=20, tile_row=20, tile_width=1, azimuth_count=1440,
+                           maximum_distance_m=1_000_000.0, observer_elevation_m=0.0,
+                           subpatch_size=8, exact_chain=True)
+    # the captured azimuth-360, DEM-1 segments (test_phase4_pyramid.py:231-235 bound)
+    np.testing.assert_allclose(segs[360, :, 1], np.asarray(fx.phase4d()["segments"], np.float32),
+                               rtol=0, atol=6e-8)
+    expected_final = arrays[f"{base}__final_slopes"][0]
+    expected_per = arrays[f"{base}__per_dem_slopes"]
+    expected_deg = arrays[f"{base}__final_degrees"][0]
+    others = np.arange(1440) != 360
+    capped, _ = oracle.horizon_tile(pyramids, segs, tile_column=20, tile_row=20, tile_width=1,
+                                    tile_height=1, mode=oracle.FLOAT32 | oracle.CAP)
+    np.testing.assert_allclose(capped[0, others], expected_final[others], rtol=0, atol=3e-7)
+    assert abs(capped[0, 360] - expected_final[360]) < 1e-3
+    pass0, _ = oracle.horizon_tile(pyramids[:1], segs[:, :, :1], tile_column=20, tile_row=20,
+                                   tile_width=1, tile_height=1, mode=oracle.FLOAT32 | oracle.CAP)
+    np.testing.assert_allclose(pass0[0], expected_per[0, 0], rtol=0, atol=3e-7)
+    method, counters = oracle.horizon_tile(pyramids, segs, tile_column=20, tile_row=20,
+                                           tile_width=1, tile_height=1, float64=True)
+    degrees = np.degrees(np.arctan(method[0]))
+    assert np.max(np.abs(degrees[others] - expected_deg[others])) < 1e-3
+    assert counters[0] > 0 and counters[1] > 0
+    # the production (uncapped) schedule steps past the spike at azimuth 360
+    assert degrees[360] < expected_deg[360]
+
+
+def test_flat_sphere_closed_form():
+    """Flat spherical terrain: a ray's max slope is the first far sample's -d/(2R)."""
+    R = 1737400.0
+    w = h = 256
+    res = 20.0
+    dem = DemGrid(np.zeros((h, w), np.float32),
+                  np.array((-w * res / 2, res, 0, h * res / 2 + 15000, 0, -res), np.float64),
+                  ProjectionParameters.south_polar(R))
+    pyr = oracle.OraclePyramid(dem)
+    segs = oracle.segments([dem], tile_column=64, tile_row=64, tile_width=128,
+                           azimuth_count=16, maximum_distance_m=1e6)
+    slopes, _ = oracle.horizon_tile([pyr], segs, tile_column=64, tile_row=64,
+                                    pixels=[0, 777, 16383], float64=True)
+    # The near field (s < 0.5 km) uses the flat slope (h - oz)/d = 0, so every ray's
+    # maximum on a bare sphere is exactly zero.
+    np.testing.assert_array_equal(slopes, 0.0)
+
+
+def test_sun_fraction_matches_csharp_builder_bytes():
+    for case in fx.lightmap_csharp()["cases"]:
+        horizon = fx.decode_f32(case["horizons_float32_base64"])
+        for sample in case["samples"]:
+            f = oracle.sun_fraction(horizon, sample["azimuth_deg"], sample["elevation_deg"])
+            assert int(np.uint8(np.float32(255.0) * np.float32(f))) == sample["encoded_byte"]
+            f64 = oracle.sun_fraction(horizon, sample["azimuth_deg"], sample["elevation_deg"],
+                                      float32=False)
+            assert abs(f64 - f) < 1e-4  # float32 vs float64 slice accumulation
+
+
+def _unit_dem(width=1, height=1):
+    return DemGrid(np.zeros((height, width), np.float32),
+                   np.asarray((1000.0, 1.0, 0.0, -1000.0, 0.0, -1.0), np.float64),
+                   ProjectionParameters.south_polar())
+
+
+def test_lighting_full_half_zero_and_strict_psr_limb():
+    dem = _unit_dem()
+    horizons = np.zeros((128, 128, 1440), np.float32)
+    vectors = np.stack([local_direction_vector(dem, 0, 0, 0.0, e, 1.5e11) for e in (1.0, 0.0, -1.0)])
+    out = oracle.lighting_patch(dem, horizons, vectors, tile_y=0, tile_x=0, valid_height=1,
+                                valid_width=1, float32_slices=True)
+    np.testing.assert_array_equal(out["bytes"][:, 0, 0], (255, 127, 0))
+    hz = np.full((128, 128, 1440), 0.3, np.float32)
+    shadow, _ = oracle.psr_patch(dem, hz, local_direction_vector(dem, 0, 0, 0.0, 0.0, 1.5e11)[None],
+                                 tile_y=0, tile_x=0, valid_height=1, valid_width=1)
+    lit, _ = oracle.psr_patch(dem, hz, local_direction_vector(dem, 0, 0, 0.0, 0.1, 1.5e11)[None],
+                              tile_y=0, tile_x=0, valid_height=1, valid_width=1)
+    assert shadow.item() == 255 and lit.item() == 0
+
+
+def test_psr_matches_csharp_kernel_fixture():
+    art = fx.psr_csharp()
+    p = art["projection"]
+    x = np.arange(128, dtype=np.float64)[None, :]
+    y = np.arange(128, dtype=np.float64)[:, None]
+    dem = DemGrid(np.ascontiguousarray((x - y) * 0.1, dtype=np.float32),
+                  np.asarray(art["geotransform"], np.float64),
+                  ProjectionParameters(p["radius_m"], p["latitude_origin_rad"],
+                                       p["longitude_origin_rad"], p["scale"],
+                                       p["false_easting_m"], p["false_northing_m"]))
+    pixel = np.arange(128 * 128, dtype=np.int32)[:, None]
+    azimuth = np.arange(1440, dtype=np.int32)[None, :]
+    for case in art["cases"]:
+        if case["name"] == "constant_shadow":
+            hz = np.full((128 * 128, 1440), 0.3, np.float32)
+        else:
+            hz = (np.float32(0.45) + np.float32(0.01) * (pixel % 23)
+                  + np.float32(0.0001) * (azimuth % 17)).astype(np.float32)
+            if case["name"] == "compressed_quantized_mixed":
+                scaled = hz * np.float32(32767.0 / 50.0)
+                hz = (np.floor(scaled + np.float32(0.5)).astype(np.int16)
+                      * np.float32(50.0 / 32767.0)).astype(np.float32)
+        expected = np.frombuffer(__import__("base64").b64decode(case["output_base64"]),
+                                 dtype=np.uint8).reshape(128, 128)
+        actual, _ = oracle.psr_patch(dem, hz.reshape(128, 128, 1440),
+                                     np.asarray(case["sun_vectors_m"], np.float64).reshape(-1, 3),
+                                     tile_y=0, tile_x=0)
+        np.testing.assert_array_equal(actual, expected, err_msg=case["name"])
+
+
+def test_psr_reduction_keeps_highest_elevation_per_bin():
+    dem = _unit_dem()
+    vectors = np.stack([local_direction_vector(dem, 0, 0, a, e, 1.5e11)
+                        for a, e in ((10.01, 1.0), (10.02, 2.0), (20.0, -1.0))])
+    reduced, idx = oracle.psr_reduce(dem, vectors)
+    np.testing.assert_array_equal(idx, (1, 2))
+    np.testing.assert_array_equal(reduced, vectors[idx])
+
+
+def test_cbin_round_trip_and_structure():
+    rng = np.random.default_rng(5)
+    deg = rng.uniform(-60, 60, (16384, 1440)).astype(np.float32)
+    deg[:, ::97] = -np.inf
+    deg[7] = 0.25
+    payload = oracle.cbin_payload(deg)
+    back = oracle.cbin_decode(payload)
+    clipped = np.clip(deg, -50, 50)
+    # deltas beyond +-16383 are clamped by the format, so compare where no clamp hit
+    q = np.round(clipped.astype(np.float64) * (32767 / 50))
+    ok = np.all(np.abs(np.diff(q, axis=1)) <= 16383, axis=1)
+    assert np.max(np.abs(back[ok] - clipped[ok])) <= 50 / 32767 / 2 + 1e-6
+    with pytest.raises(ValueError):
+        oracle.cbin_decode(payload[:-1])
+    with pytest.raises(ValueError):
+        oracle.cbin_decode(payload + b"\x00")
