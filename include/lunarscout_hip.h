@@ -1,0 +1,143 @@
+/*
+ * lunarscout_hip.h -- C ABI of the MI355X horizon and lighting library
+ * (liblunarscout_hip.so, built for gfx950).
+ *
+ * Plain C: pointers, sizes and status codes; no C++ or torch types. Every entry
+ * point returns LSH_OK (0) or a negative status; lsh_last_error() gives the
+ * calling thread's last message. Output buffers are not written when argument
+ * validation fails. Buffers marked "device" are device memory obtained from
+ * lsh_alloc (or any HIP device allocation on the session's device); the library
+ * never frees caller memory.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths under src/lunarscout/_numba_horizon/ of the reference repository).
+ */
+#ifndef LUNARSCOUT_HIP_H
+#define LUNARSCOUT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. LSH_EINVAL maps to ValueError (cuda_backend.py:1459-1465);
+ * LSH_ENODEV to the backend-unavailable error (CudaBackendError,
+ * cuda_backend.py:1164-1176); LSH_EDEVICE to an execution failure
+ * (horizon.py:426-431). */
+#define LSH_OK 0
+#define LSH_EINVAL -1
+#define LSH_ENODEV -2
+#define LSH_EDEVICE -3
+#define LSH_ENOMEM -4
+
+/* lsh_horizon_tile / lsh_horizon_batch flags */
+#define LSH_HORIZON_DEGREES 1   /* write float32 degrees (contract.py:362-371) instead of slopes */
+#define LSH_HORIZON_CAP 2       /* diagnostic: cap level-0 advances at the cell exit (hierarchy.py:258-262) */
+#define LSH_HORIZON_COUNTERS 4  /* accumulate traversal counters (hierarchy.py:40-52) */
+
+/* lsh_lighting_tiles flags */
+#define LSH_LIGHT_SUN_BYTES 1     /* per-time uint8 sun fraction bytes */
+#define LSH_LIGHT_SUN_FRACTION 2  /* per-time float32 sun fraction */
+#define LSH_LIGHT_SUN_MEAN 4      /* per-pixel mean sun fraction over all times */
+#define LSH_LIGHT_EARTH_MARGIN 8  /* per-time float32 Earth margin */
+#define LSH_LIGHT_EARTH_VISIBLE 16/* per-pixel fraction of times with Earth margin >= 0 */
+#define LSH_LIGHT_PSR 32          /* per-pixel PSR mask (255 = PSR) */
+#define LSH_LIGHT_PSR_MARGIN 64   /* per-pixel float32 max(elevation - horizon) over PSR vectors */
+
+typedef struct lsh_session lsh_session;
+
+/* A horizon tile origin within the primary DEM. */
+typedef struct {
+  int32_t tile_column;
+  int32_t tile_row;
+} lsh_tile;
+
+/* Thread-local text of the last error. Returns the message length. */
+int lsh_last_error(char* buffer, size_t capacity);
+
+/* Library version string "major.minor.patch". */
+const char* lsh_version(void);
+
+/* Number of visible HIP devices (0 when none). Replaces the device probe of
+ * cuda.py:14-138 / CudaSession.__init__ (cuda_backend.py:1160-1176). */
+int lsh_device_count(int32_t* count);
+
+/* Device name and compute-unit count. */
+int lsh_device_info(int32_t device, char* name, size_t capacity, int32_t* compute_units);
+
+/* Session: one device, one stream, resident DEMs and reusable scratch
+ * (CudaSession, cuda_backend.py:1157-1258). */
+int lsh_session_create(int32_t device, lsh_session** session);
+int lsh_session_destroy(lsh_session* session);
+
+/* Device memory helpers (the reference's cuda.device_array / to_device /
+ * copy_to_host). Copies are synchronous with respect to the session stream. */
+int lsh_alloc(lsh_session* session, size_t bytes, void** device_pointer);
+int lsh_free(lsh_session* session, void* device_pointer);
+int lsh_copy_to_device(lsh_session* session, void* device_dst, const void* host_src, size_t bytes);
+int lsh_copy_to_host(lsh_session* session, void* host_dst, const void* device_src, size_t bytes);
+int lsh_memset_device(lsh_session* session, void* device_dst, int value, size_t bytes);
+int lsh_synchronize(lsh_session* session);
+
+/* Register a DEM on the device and build its factor-four max pyramid there
+ * (CudaSession.build_max_pyramid, cuda_backend.py:1260-1281, and the resident
+ * pyramids of _prepare_production_pyramids, cuda_backend.py:1231-1258).
+ * elevation: host float32[height][width]; geo_transform: 6 float64
+ * (origin_x, t1, t2, origin_y, t4, t5); projection: 6 float64 (radius,
+ * lat0, lon0, scale, false_easting, false_northing). mips_host (nullable):
+ * a cached pyramid payload (pyramid.py:34-48) used instead of building. */
+int lsh_dem_create(lsh_session* session, const float* elevation, int32_t width, int32_t height,
+                   const double* geo_transform, const double* projection,
+                   const float* mips_host, int32_t* dem_id);
+int lsh_dem_destroy(lsh_session* session, int32_t dem_id);
+
+/* Pyramid layout of a width x height DEM (pyramid.py:19-31): levels int32[L][4]
+ * = (level, offset, width, height); capacity in rows. */
+int lsh_pyramid_layout(int32_t width, int32_t height, int32_t* levels, int32_t capacity,
+                       int32_t* level_count, int64_t* mip_count);
+
+/* Copy a registered DEM's mips to the host (the .pyr.bin payload,
+ * pyramid.py:56-75). mips: float32[mip_count]. */
+int lsh_dem_pyramid(lsh_session* session, int32_t dem_id, float* mips_host, int64_t capacity);
+
+/* Device pointers of a registered DEM (level 0 and mips), for callers that
+ * build their own kernels' inputs. */
+int lsh_dem_device_pointers(lsh_session* session, int32_t dem_id, const float** level0,
+                            const float** mips);
+
+/* FP64 ray-origin, direction, chord sampling and segment fitting of one tile on
+ * the GPU (build_subpatch_segments_numba, geometry.py:575-673, with
+ * generate_segments_parallel, geometry_numba.py:352-364).
+ * segments (device): float32[N][C][D][18], C = (tile_width/subpatch + 2)^2. */
+int lsh_segments_tile(lsh_session* session, const int32_t* dem_ids, int32_t dem_count,
+                      int32_t tile_column, int32_t tile_row, int32_t tile_width,
+                      int32_t azimuth_count, double maximum_distance_m,
+                      double observer_elevation_m, int32_t subpatch_size, float* segments);
+
+/* Every DEM pass of one horizon tile in a single launch, the running maximum
+ * carried in registers (CudaSession.subpatch_hierarchical_all_passes,
+ * cuda_backend.py:1445-1552, and subpatch_hierarchy_kernel,
+ * cuda_backend.py:1007-1134). segments (device): float32[N][C][D][18];
+ * output (device): float32[tile_height*tile_width][N], pixel-major; pixels
+ * outside the primary DEM are -inf. counters (device, nullable, with
+ * LSH_HORIZON_COUNTERS): int64[3] += (level-0 samples, bound tests, steps). */
+int lsh_horizon_tile(lsh_session* session, const int32_t* dem_ids, int32_t dem_count,
+                     const float* segments, int32_t azimuth_count, int32_t tile_column,
+                     int32_t tile_row, int32_t tile_width, int32_t tile_height,
+                     int32_t subpatch_size, float observer_elevation_m, int32_t flags,
+                     float* output, int64_t* counters);
+
+/* Segments and march of one tile in one call (lsh_segments_tile followed by
+ * lsh_horizon_tile) using session-owned segment scratch. */
+int lsh_horizon_tile_full(lsh_session* session, const int32_t* dem_ids, int32_t dem_count,
+                          int32_t tile_column, int32_t tile_row, int32_t azimuth_count,
+                          double maximum_distance_m, float observer_elevation_m,
+                          int32_t flags, float* output, int64_t* counters);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LUNARSCOUT_HIP_H */

@@ -1,0 +1,423 @@
+// lsh_abi.hip -- the C ABI: sessions, device memory, DEM registration and the
+// horizon entry points. Declarations and reference citations are in
+// include/lunarscout_hip.h.
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "lsh_internal.h"
+
+namespace lsh {
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string& message) { g_error = message; }
+
+}  // namespace lsh
+
+using lsh::Dem;
+using lsh::Session;
+using lsh::set_error;
+
+#define LSH_CHECK_HIP(expr, what)                                              \
+  do {                                                                         \
+    hipError_t _e = (expr);                                                    \
+    if (_e != hipSuccess) {                                                    \
+      set_error(std::string(what) + ": " + hipGetErrorString(_e));             \
+      return _e == hipErrorOutOfMemory ? LSH_ENOMEM : LSH_EDEVICE;             \
+    }                                                                          \
+  } while (0)
+
+#define LSH_REQUIRE(cond, message) \
+  do {                             \
+    if (!(cond)) {                 \
+      set_error(message);          \
+      return LSH_EINVAL;           \
+    }                              \
+  } while (0)
+
+namespace {
+
+int pyramid_layout(int w, int h, int32_t (*levels)[4], int cap, int64_t* mip_count) {
+  int count = 0;
+  int64_t offset = 0;
+  int cw = w, ch = h;
+  while (true) {
+    if (count >= cap) return -1;
+    levels[count][0] = count;
+    levels[count][1] = count == 0 ? 0 : static_cast<int32_t>(offset);
+    levels[count][2] = cw;
+    levels[count][3] = ch;
+    if (count > 0) offset += static_cast<int64_t>(cw) * ch;
+    ++count;
+    if (cw == 1 && ch == 1) break;
+    cw = (cw + 3) / 4;
+    ch = (ch + 3) / 4;
+  }
+  *mip_count = offset;
+  return count;
+}
+
+Dem* find_dem(Session* s, int32_t id) {
+  if (!s || id < 0 || id >= static_cast<int32_t>(s->dems.size()) || !s->dems[id].live) return nullptr;
+  return &s->dems[id];
+}
+
+int select_device(Session* s) {
+  LSH_CHECK_HIP(hipSetDevice(s->device), "hipSetDevice");
+  return LSH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lsh_last_error(char* buffer, size_t capacity) {
+  const std::string& e = lsh::g_error;
+  if (buffer && capacity) {
+    size_t n = e.size() < capacity - 1 ? e.size() : capacity - 1;
+    std::memcpy(buffer, e.data(), n);
+    buffer[n] = 0;
+  }
+  return static_cast<int>(e.size());
+}
+
+const char* lsh_version(void) { return "0.1.0"; }
+
+int lsh_device_count(int32_t* count) {
+  LSH_REQUIRE(count, "count must not be null");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  *count = e == hipSuccess ? n : 0;
+  return LSH_OK;
+}
+
+int lsh_device_info(int32_t device, char* name, size_t capacity, int32_t* compute_units) {
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) {
+    set_error(std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+    return LSH_ENODEV;
+  }
+  if (name && capacity) {
+    std::snprintf(name, capacity, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (compute_units) *compute_units = prop.multiProcessorCount;
+  return LSH_OK;
+}
+
+int lsh_session_create(int32_t device, lsh_session** session) {
+  LSH_REQUIRE(session, "session must not be null");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    set_error("no HIP device is visible");
+    return LSH_ENODEV;
+  }
+  if (device < 0 || device >= n) {
+    set_error("HIP device index out of range");
+    return LSH_ENODEV;
+  }
+  Session* s = new Session();
+  s->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    set_error(std::string("cannot initialise HIP device: ") + hipGetErrorString(e));
+    delete s;
+    return LSH_ENODEV;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) s->cu_count = prop.multiProcessorCount;
+  *session = reinterpret_cast<lsh_session*>(s);
+  return LSH_OK;
+}
+
+int lsh_session_destroy(lsh_session* session) {
+  Session* s = reinterpret_cast<Session*>(session);
+  if (!s) return LSH_OK;
+  hipSetDevice(s->device);
+  hipStreamSynchronize(s->stream);
+  for (Dem& d : s->dems) {
+    if (d.live) {
+      hipFree(d.level0);
+      hipFree(d.mips);
+    }
+  }
+  if (s->segment_scratch) hipFree(s->segment_scratch);
+  hipStreamDestroy(s->stream);
+  delete s;
+  return LSH_OK;
+}
+
+int lsh_alloc(lsh_session* session, size_t bytes, void** device_pointer) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(s && device_pointer, "invalid arguments");
+  if (int r = select_device(s)) return r;
+  LSH_CHECK_HIP(hipMalloc(device_pointer, bytes ? bytes : 1), "hipMalloc");
+  return LSH_OK;
+}
+
+int lsh_free(lsh_session* session, void* device_pointer) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(s, "invalid session");
+  if (int r = select_device(s)) return r;
+  LSH_CHECK_HIP(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  LSH_CHECK_HIP(hipFree(device_pointer), "hipFree");
+  return LSH_OK;
+}
+
+int lsh_copy_to_device(lsh_session* session, void* dst, const void* src, size_t bytes) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(s && dst && src, "invalid arguments");
+  if (int r = select_device(s)) return r;
+  LSH_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream), "copy to device");
+  LSH_CHECK_HIP(hipStreamSynchronize(s->stream), "copy to device");
+  return LSH_OK;
+}
+
+int lsh_copy_to_host(lsh_session* session, void* dst, const void* src, size_t bytes) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(s && dst && src, "invalid arguments");
+  if (int r = select_device(s)) return r;
+  LSH_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream), "copy to host");
+  LSH_CHECK_HIP(hipStreamSynchronize(s->stream), "copy to host");
+  return LSH_OK;
+}
+
+int lsh_memset_device(lsh_session* session, void* dst, int value, size_t bytes) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(s && dst, "invalid arguments");
+  if (int r = select_device(s)) return r;
+  LSH_CHECK_HIP(hipMemsetAsync(dst, value, bytes, s->stream), "hipMemsetAsync");
+  return LSH_OK;
+}
+
+int lsh_synchronize(lsh_session* session) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(s, "invalid session");
+  if (int r = select_device(s)) return r;
+  LSH_CHECK_HIP(hipStreamSynchronize(s->stream), "kernel execution");
+  return LSH_OK;
+}
+
+int lsh_pyramid_layout(int32_t width, int32_t height, int32_t* levels, int32_t capacity,
+                       int32_t* level_count, int64_t* mip_count) {
+  LSH_REQUIRE(width > 0 && height > 0, "DEM dimensions must be positive");
+  LSH_REQUIRE(level_count && mip_count, "invalid arguments");
+  int32_t table[lsh::kMaxLevels][4];
+  int64_t mips = 0;
+  int n = pyramid_layout(width, height, table, lsh::kMaxLevels, &mips);
+  LSH_REQUIRE(n > 0, "DEM too large for the pyramid table");
+  if (levels) {
+    LSH_REQUIRE(capacity >= n, "level table capacity too small");
+    std::memcpy(levels, table, sizeof(int32_t) * 4 * n);
+  }
+  *level_count = n;
+  *mip_count = mips;
+  return LSH_OK;
+}
+
+int lsh_dem_create(lsh_session* session, const float* elevation, int32_t width, int32_t height,
+                   const double* geo_transform, const double* projection,
+                   const float* mips_host, int32_t* dem_id) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(s && elevation && geo_transform && projection && dem_id, "invalid arguments");
+  LSH_REQUIRE(width > 0 && height > 0, "DEM dimensions must be positive");
+  double det = geo_transform[1] * geo_transform[5] - geo_transform[2] * geo_transform[4];
+  LSH_REQUIRE(std::fabs(det) >= 1e-12, "DEM geotransform is singular");
+  for (int i = 0; i < 6; ++i)
+    LSH_REQUIRE(std::isfinite(geo_transform[i]) && std::isfinite(projection[i]),
+                "DEM transform and projection must be finite");
+  Dem d;
+  int n = pyramid_layout(width, height, d.levels, lsh::kMaxLevels, &d.mip_count);
+  LSH_REQUIRE(n > 0, "DEM too large for the pyramid table");
+  d.level_count = n;
+  d.width = width;
+  d.height = height;
+  std::memcpy(d.geo_transform, geo_transform, sizeof(d.geo_transform));
+  std::memcpy(d.projection, projection, sizeof(d.projection));
+  if (int r = select_device(s)) return r;
+  std::lock_guard<std::mutex> lock(s->mutex);
+  size_t level0_bytes = sizeof(float) * static_cast<size_t>(width) * height;
+  size_t mip_bytes = sizeof(float) * static_cast<size_t>(d.mip_count > 0 ? d.mip_count : 1);
+  LSH_CHECK_HIP(hipMalloc(&d.level0, level0_bytes), "hipMalloc DEM");
+  hipError_t e = hipMalloc(&d.mips, mip_bytes);
+  if (e != hipSuccess) {
+    hipFree(d.level0);
+    set_error(std::string("hipMalloc pyramid: ") + hipGetErrorString(e));
+    return LSH_ENOMEM;
+  }
+  e = hipMemcpyAsync(d.level0, elevation, level0_bytes, hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess && mips_host && d.mip_count > 0)
+    e = hipMemcpyAsync(d.mips, mips_host, sizeof(float) * d.mip_count, hipMemcpyHostToDevice,
+                       s->stream);
+  if (e == hipSuccess && !mips_host) {
+    const float* src = d.level0;
+    int sw = width, sh = height;
+    for (int l = 1; l < n && e == hipSuccess; ++l) {
+      float* dst = d.mips + d.levels[l][1];
+      e = lsh::launch_pyramid_level(src, sw, sh, dst, d.levels[l][2], d.levels[l][3], s->stream);
+      src = dst;
+      sw = d.levels[l][2];
+      sh = d.levels[l][3];
+    }
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) {
+    hipFree(d.level0);
+    hipFree(d.mips);
+    set_error(std::string("DEM upload / pyramid build: ") + hipGetErrorString(e));
+    return LSH_EDEVICE;
+  }
+  d.live = true;
+  s->dems.push_back(d);
+  *dem_id = static_cast<int32_t>(s->dems.size() - 1);
+  return LSH_OK;
+}
+
+int lsh_dem_destroy(lsh_session* session, int32_t dem_id) {
+  Session* s = reinterpret_cast<Session*>(session);
+  Dem* d = find_dem(s, dem_id);
+  LSH_REQUIRE(d, "unknown DEM id");
+  if (int r = select_device(s)) return r;
+  hipStreamSynchronize(s->stream);
+  hipFree(d->level0);
+  hipFree(d->mips);
+  d->live = false;
+  return LSH_OK;
+}
+
+int lsh_dem_pyramid(lsh_session* session, int32_t dem_id, float* mips_host, int64_t capacity) {
+  Session* s = reinterpret_cast<Session*>(session);
+  Dem* d = find_dem(s, dem_id);
+  LSH_REQUIRE(d && mips_host, "invalid arguments");
+  LSH_REQUIRE(capacity >= d->mip_count, "mip buffer too small");
+  if (d->mip_count == 0) return LSH_OK;
+  return lsh_copy_to_host(session, mips_host, d->mips, sizeof(float) * d->mip_count);
+}
+
+int lsh_dem_device_pointers(lsh_session* session, int32_t dem_id, const float** level0,
+                            const float** mips) {
+  Session* s = reinterpret_cast<Session*>(session);
+  Dem* d = find_dem(s, dem_id);
+  LSH_REQUIRE(d, "unknown DEM id");
+  if (level0) *level0 = d->level0;
+  if (mips) *mips = d->mips;
+  return LSH_OK;
+}
+
+int lsh_segments_tile(lsh_session* session, const int32_t* dem_ids, int32_t dem_count,
+                      int32_t tile_column, int32_t tile_row, int32_t tile_width,
+                      int32_t azimuth_count, double maximum_distance_m,
+                      double observer_elevation_m, int32_t subpatch_size, float* segments) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(s && dem_ids && segments, "invalid arguments");
+  LSH_REQUIRE(dem_count >= 1 && dem_count <= lsh::kMaxDems, "dem_count must be 1..4");
+  LSH_REQUIRE(azimuth_count > 0, "azimuth_count must be positive");
+  LSH_REQUIRE(subpatch_size > 0 && tile_width > 0, "tile width and subpatch size must be positive");
+  LSH_REQUIRE(std::isfinite(maximum_distance_m) && std::isfinite(observer_elevation_m),
+              "distances must be finite");
+  lsh::SegmentArgs a{};
+  a.out = segments;
+  a.N = azimuth_count;
+  a.count = tile_width / subpatch_size + 2;
+  a.C = a.count * a.count;
+  a.D = dem_count;
+  a.tile_column = tile_column;
+  a.tile_row = tile_row;
+  a.subpatch = subpatch_size;
+  a.maximum_distance = maximum_distance_m;
+  a.observer_elevation = observer_elevation_m;
+  for (int i = 0; i < dem_count; ++i) {
+    Dem* d = find_dem(s, dem_ids[i]);
+    LSH_REQUIRE(d, "unknown DEM id");
+    lsh::SegmentDem& g = a.dem[i];
+    g.elevation = d->level0;
+    g.width = d->width;
+    g.height = d->height;
+    std::memcpy(g.t, d->geo_transform, sizeof(g.t));
+    std::memcpy(g.p, d->projection, sizeof(g.p));
+    // DemGrid.map_resolution_m and the ray limit (geometry.py:78-83, 452-462)
+    g.map_resolution = (std::hypot(g.t[1], g.t[4]) + std::hypot(g.t[2], g.t[5])) * 0.5;
+    double size_m = std::fmin(d->width * g.map_resolution, d->height * g.map_resolution);
+    g.ray_limit = std::fmin(maximum_distance_m, size_m * 1.2);
+  }
+  if (int r = select_device(s)) return r;
+  LSH_CHECK_HIP(lsh::launch_segments(a, s->stream), "segment kernel launch");
+  return LSH_OK;
+}
+
+int lsh_horizon_tile(lsh_session* session, const int32_t* dem_ids, int32_t dem_count,
+                     const float* segments, int32_t azimuth_count, int32_t tile_column,
+                     int32_t tile_row, int32_t tile_width, int32_t tile_height,
+                     int32_t subpatch_size, float observer_elevation_m, int32_t flags,
+                     float* output, int64_t* counters) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(s && dem_ids && segments && output, "invalid arguments");
+  LSH_REQUIRE(dem_count >= 1 && dem_count <= lsh::kMaxDems, "dem_count must be 1..4");
+  LSH_REQUIRE(azimuth_count > 0, "azimuth_count must be positive");
+  LSH_REQUIRE(tile_width > 0 && tile_height > 0 && tile_width <= 128 && tile_height <= 128,
+              "tile dimensions must be 1..128");
+  LSH_REQUIRE(subpatch_size > 0 && tile_width / subpatch_size + 2 >= 2, "invalid subpatch size");
+  LSH_REQUIRE(std::isfinite(observer_elevation_m), "observer elevation must be finite");
+  lsh::MarchArgs a{};
+  a.segments = segments;
+  a.output = output;
+  a.winners = nullptr;
+  a.counters = reinterpret_cast<unsigned long long*>(counters);
+  a.N = azimuth_count;
+  int count = tile_width / subpatch_size + 2;
+  a.C = count * count;
+  a.D = dem_count;
+  a.tile_column = tile_column;
+  a.tile_row = tile_row;
+  a.tile_width = tile_width;
+  a.tile_height = tile_height;
+  a.subpatch = subpatch_size;
+  a.observer_elevation = observer_elevation_m;
+  a.flags = flags;
+  for (int i = 0; i < dem_count; ++i) {
+    Dem* d = find_dem(s, dem_ids[i]);
+    LSH_REQUIRE(d, "unknown DEM id");
+    a.dem[i] = lsh::march_dem(*d);
+  }
+  LSH_REQUIRE(tile_column >= 0 && tile_row >= 0 && tile_column < a.dem[0].width &&
+                  tile_row < a.dem[0].height,
+              "tile origin lies outside the primary DEM");
+  if (int r = select_device(s)) return r;
+  LSH_CHECK_HIP(lsh::launch_march(a, s->stream), "march kernel launch");
+  return LSH_OK;
+}
+
+int lsh_horizon_tile_full(lsh_session* session, const int32_t* dem_ids, int32_t dem_count,
+                          int32_t tile_column, int32_t tile_row, int32_t azimuth_count,
+                          double maximum_distance_m, float observer_elevation_m,
+                          int32_t flags, float* output, int64_t* counters) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(s, "invalid session");
+  LSH_REQUIRE(dem_count >= 1 && dem_count <= lsh::kMaxDems, "dem_count must be 1..4");
+  LSH_REQUIRE(azimuth_count > 0, "azimuth_count must be positive");
+  size_t bytes = sizeof(float) * static_cast<size_t>(azimuth_count) * 324 * dem_count * 18;
+  if (int r = select_device(s)) return r;
+  if (s->segment_scratch_bytes < bytes) {
+    LSH_CHECK_HIP(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    if (s->segment_scratch) hipFree(s->segment_scratch);
+    s->segment_scratch = nullptr;
+    s->segment_scratch_bytes = 0;
+    LSH_CHECK_HIP(hipMalloc(&s->segment_scratch, bytes), "hipMalloc segment scratch");
+    s->segment_scratch_bytes = bytes;
+  }
+  int r = lsh_segments_tile(session, dem_ids, dem_count, tile_column, tile_row, 128,
+                            azimuth_count, maximum_distance_m, observer_elevation_m, 8,
+                            s->segment_scratch);
+  if (r) return r;
+  return lsh_horizon_tile(session, dem_ids, dem_count, s->segment_scratch, azimuth_count,
+                          tile_column, tile_row, 128, 128, 8, observer_elevation_m, flags,
+                          output, counters);
+}
+
+}  // extern "C"

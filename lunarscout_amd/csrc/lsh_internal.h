@@ -1,0 +1,98 @@
+// lsh_internal.h -- shared types of the HIP library (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/lunarscout_hip.h"
+
+namespace lsh {
+
+constexpr int kMaxDems = 4;        // DEM passes fused in one march launch
+constexpr int kMaxLevels = 16;     // pyramid levels (a 4^15-pixel DEM)
+constexpr int kSegmentFields = 18; // SEGMENT_FIELDS (contract.py:33-38)
+
+// Thread-local error text set by every failing entry point.
+void set_error(const std::string& message);
+
+struct Dem {
+  bool live = false;
+  int32_t width = 0, height = 0;
+  double geo_transform[6] = {0};
+  double projection[6] = {0};
+  float* level0 = nullptr;  // device float32[h][w]
+  float* mips = nullptr;    // device float32[mip_count] (levels 1..L-1)
+  int64_t mip_count = 0;
+  int32_t level_count = 0;
+  int32_t levels[kMaxLevels][4] = {{0}};  // (level, offset, width, height)
+};
+
+// Per-pass description of a DEM as the march kernel reads it.
+struct MarchDem {
+  const float* level0;
+  const float* mips;
+  int32_t width, height, level_count;
+  float map_resolution;     // (column + row resolution) / 2 (cuda_backend.py:1124-1127)
+  float column_resolution;  // hypot(t1, t4) (cuda_backend.py:1032-1035)
+  float radius;             // projection radius (cuda_backend.py:1132)
+  int32_t level_offset[kMaxLevels];
+  int32_t level_width[kMaxLevels];
+  int32_t level_height[kMaxLevels];
+};
+
+struct MarchArgs {
+  const float* segments;  // [N][C][D][18]
+  float* output;          // [tile_h * tile_w][N]
+  int32_t* winners;       // nullable [tile_h * tile_w][N] winning-sample index
+  unsigned long long* counters;  // nullable [3]
+  int32_t N, C, D;
+  int32_t tile_column, tile_row, tile_width, tile_height, subpatch;
+  float observer_elevation;
+  int32_t flags;
+  MarchDem dem[kMaxDems];
+};
+
+struct Session {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mutex;
+  std::vector<Dem> dems;
+  // session-owned scratch for the one-call tile path
+  float* segment_scratch = nullptr;
+  size_t segment_scratch_bytes = 0;
+  int cu_count = 0;
+};
+
+// Kernel launchers (defined in the .hip files).
+hipError_t launch_pyramid_level(const float* src, int sw, int sh, float* dst, int dw, int dh,
+                                hipStream_t stream);
+
+struct SegmentDem {
+  const float* elevation;  // device level 0
+  int32_t width, height;
+  double t[6];
+  double p[6];
+  double map_resolution;
+  double ray_limit;
+};
+
+struct SegmentArgs {
+  float* out;  // [N][C][D][18]
+  int32_t N, C, D, count;  // count = tile_width/subpatch + 2
+  int32_t tile_column, tile_row, subpatch;
+  double maximum_distance;
+  double observer_elevation;
+  SegmentDem dem[kMaxDems];
+};
+
+hipError_t launch_segments(const SegmentArgs& args, hipStream_t stream);
+hipError_t launch_march(const MarchArgs& args, hipStream_t stream);
+
+// Fills a MarchDem from a registered DEM (float32 parameters the kernel reads).
+MarchDem march_dem(const Dem& dem);
+
+}  // namespace lsh

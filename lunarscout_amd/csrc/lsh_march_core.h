@@ -1,0 +1,403 @@
+// lsh_march_core.h -- the per-ray horizon march shared by the HIP kernels.
+//
+// Restates hierarchy_maximum (cuda_backend.py:656-874) and the per-pixel
+// subpatch interpolation (cuda_backend.py:1040-1123, 965-987) as __host__
+// __device__ code. The same source also compiles with a host C++ compiler (a
+// CPU build of the kernel arithmetic used by the tests to check the GPU
+// bit for bit); the product library only compiles it for the GPU.
+//
+// The step schedule is the reference's; the precision is split in two:
+//   * the schedule state -- ray distance s, pixel position, tangent, cell-exit
+//     distances and step lengths -- is evaluated in precision S (float64 in the
+//     product kernel). Samples land wherever the schedule puts them, and a
+//     sample that lands within float32 rounding (~1e-7 px) of a cell edge, or a
+//     step that drifts by float32 rounding, sends a float32 schedule down another
+//     branch than the float64 method: on rough terrain the sampling phase then
+//     shifts by up to half a pixel and the horizon by up to several degrees;
+//   * heights, bounds and slopes are evaluated in precision V (float32 in the
+//     product kernel), arranged so rounding stays small where it matters:
+//     pixel coordinates are carried relative to an integer base near the ray
+//     start; heights enter the slope as differences to the observer, h - oz,
+//     taken on the four DEM taps before the bilinear blend; and the level-0
+//     slope uses (h - oz)(2R + h + oz), the form the reference already uses for
+//     its cull bound (cuda_backend.py:777-795), instead of rounding R + h to
+//     float32 (0.125 m) first. With S = float32 the ray distance is kept as an
+//     unevaluated float32 pair (s_hi + s_lo).
+// Each rearrangement is algebraically exact; decisions, constants and the
+// order in which samples and bound tests are taken are unchanged.
+
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define LSH_HD __host__ __device__ inline
+#define LSH_HD_MEMBER __host__ __device__
+#else
+#define LSH_HD inline
+#define LSH_HD_MEMBER
+#endif
+
+namespace lsh {
+
+constexpr float kBeamWidth = 0.004363323129985824f;  // 2*pi/1440 (cuda_backend.py:699)
+
+struct MarchView {
+  const float* level0;
+  const float* mips;
+  const int32_t* level_offset;
+  const int32_t* level_width;
+  const int32_t* level_height;
+  int32_t width, height, level_count;
+  float map_resolution;  // (column + row resolution) / 2
+  float radius;
+};
+
+// One pass's interpolated segment with its start relative to (bx, by), in the
+// schedule precision S.
+template <class S>
+struct PassSegment {
+  int32_t bx, by;
+  S sx, sy;
+  S a1, a2, a3, a4, b1, b2, b3, b4;
+  S s_start, s_end, s_chord, c1, c2, c3;
+};
+
+struct MarchCounters {
+  uint32_t samples, tests, steps;
+};
+
+// Step recorder of the march; the kernels use NoTrace, which compiles away.
+// A row holds the reference's trace fields (hierarchy.py:29-52): s_km, true_m,
+// level, cell_x, cell_y, pixel_x, pixel_y, bound, sample h - oz, slope,
+// advance_km, action (0 descend, 1 cull, 2 nodata, 4 sample).
+struct NoTrace {
+  LSH_HD_MEMBER void row(float, float, int, int, int, float, float, float, float, float, float,
+                         int) const {}
+};
+
+LSH_HD bool lsh_valid(float v) {
+  return !__builtin_isnan(v) && !__builtin_isinf(v) && v > -20000.0f;  // cuda_backend.py:43-45
+}
+
+// evaluate (cuda_backend.py:62-71) on the start offset.
+template <class S>
+LSH_HD S lsh_quartic(S x0, S a1, S a2, S a3, S a4, S d) {
+  S d2 = d * d, d3 = d2 * d, d4 = d2 * d2;
+  S r = x0 + a1 * d;
+  r = r + a2 * d2;
+  r = r + a3 * d3;
+  return r + a4 * d4;
+}
+
+// tangent (cuda_backend.py:73-84)
+template <class S>
+LSH_HD S lsh_tangent(S a1, S a2, S a3, S a4, S d) {
+  S d2 = d * d, d3 = d2 * d;
+  S r = a1 + S(2) * a2 * d;
+  r = r + S(3) * a3 * d2;
+  return r + S(4) * a4 * d3;
+}
+
+template <class T>
+LSH_HD T lsh_sqrt(T x) {
+  return sizeof(T) == 4 ? static_cast<T>(__builtin_sqrtf(static_cast<float>(x)))
+                        : static_cast<T>(__builtin_sqrt(static_cast<double>(x)));
+}
+template <class T>
+LSH_HD T lsh_floor(T x) {
+  return sizeof(T) == 4 ? static_cast<T>(__builtin_floorf(static_cast<float>(x)))
+                        : static_cast<T>(__builtin_floor(static_cast<double>(x)));
+}
+template <class T>
+LSH_HD T lsh_max(T a, T b) { return a > b ? a : (b > a ? b : a); }
+template <class T>
+LSH_HD T lsh_min(T a, T b) { return a < b ? a : (b < a ? b : a); }
+template <class T>
+LSH_HD T lsh_abs(T a) { return a < T(0) ? -a : a; }
+
+// Elevation slope, on a sphere of radius R, of a point hd metres above an observer
+// at height oz and `distance` metres away (cuda_backend.py:777-795, 812-830).
+template <class V>
+LSH_HD V lsh_sphere_slope(V hd, V oz, V distance, V radius, V two_observer_radius) {
+  V sq = distance * distance;
+  V rs = (V(2) * radius + V(2) * oz) + hd;  // 2R + h + oz with h = oz + hd
+  V lz = (hd * rs - sq) / two_observer_radius;
+  V lxx = sq - lz * lz;
+  V lx = lxx > V(0) ? lsh_sqrt(lxx) : V(1e-6);
+  return lz / lx;
+}
+
+// Ray distance s in km. Single precision keeps an unevaluated pair (hi + lo)
+// updated by TwoSum; double precision keeps one value.
+template <class S>
+struct RayDistance {
+  S hi, lo;
+  LSH_HD void start(S v) { hi = v; lo = S(0); }
+  LSH_HD void add(S a) {
+    if (sizeof(S) == 8) {
+      hi = hi + a;
+      return;
+    }
+    S t = hi + a;
+    S bp = t - hi;
+    S e = (hi - (t - bp)) + (a - bp);
+    lo = lo + e;
+    hi = t + lo;
+    lo = lo - (hi - t);
+  }
+  LSH_HD bool at_most(S v) const { return hi < v || (hi == v && lo <= S(0)); }
+  LSH_HD bool below(S v) const { return hi < v || (hi == v && lo < S(0)); }
+  LSH_HD S minus(S v) const { return (hi - v) + lo; }
+  LSH_HD S value() const { return hi + lo; }
+};
+
+// hierarchy_maximum (cuda_backend.py:656-874) for one DEM pass. S is the
+// precision of the schedule state (ray distance, position, tangent, distances,
+// steps); V the precision of heights and slopes. `current` is the running
+// maximum slope (-1e30 when none yet); `best` receives
+// (pass << 24) | level-0 sample index when a sample raises it.
+template <class S, class V, bool kCap, bool kCount, class Tracer = NoTrace>
+LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, int pass,
+                           V& current, int32_t& best, MarchCounters& cnt,
+                           const Tracer& trace = Tracer()) {
+  const S res = static_cast<S>(d.map_resolution);
+  const V radius = static_cast<V>(d.radius);
+  const V two_observer_radius = V(2) * (radius + oz);
+  const S min_step = (S(0.5) * res) * S(0.001);
+  const S far_step = (S(0.8) * res) * S(0.001);
+  // bounds of the reference's absolute test px in [0, W-1), as offsets
+  const S lo_x = static_cast<S>(-g.bx), lo_y = static_cast<S>(-g.by);
+  const S hi_x = static_cast<S>(d.width - 1 - g.bx);
+  const S hi_y = static_cast<S>(d.height - 1 - g.by);
+  // the bilinear clamp at width - 1.0001 (cuda_backend.py:113-114), as a fraction
+  const S clamp_tx = (static_cast<S>(d.width) - S(1.0001)) - static_cast<S>(d.width - 2);
+  const S clamp_ty = (static_cast<S>(d.height) - S(1.0001)) - static_cast<S>(d.height - 2);
+  RayDistance<S> s;
+  s.start(lsh_max(g.s_start, S(0.001)));
+  bool need_position = true;
+  int level = 0, sample_index = 0;
+  S px = 0, py = 0, true_m = 0, dx = 0, dy = 0, inv_x = 0, inv_y = 0;
+  int ix = 0, iy = 0;
+  bool near = true;
+  while (true) {
+    if (need_position) {
+      if (!s.at_most(g.s_end)) break;
+      S delta = s.minus(g.s_start);
+      px = lsh_quartic(g.sx, g.a1, g.a2, g.a3, g.a4, delta);
+      py = lsh_quartic(g.sy, g.b1, g.b2, g.b3, g.b4, delta);
+      if (px != px || py != py || px < lo_x || py < lo_y || px >= hi_x || py >= hi_y) break;
+      near = s.below(S(0.5));
+      if (near) {
+        true_m = s.hi * S(1000) + s.lo * S(1000);
+      } else {
+        S plx = (px - g.sx) * res, ply = (py - g.sy) * res;
+        S planar = lsh_sqrt(plx * plx + ply * ply);
+        S p2 = planar * planar, p3 = p2 * planar;
+        S chord = g.c1 * planar;
+        chord = chord + g.c2 * p2;
+        chord = chord + g.c3 * p3;
+        true_m = g.s_chord * S(1000) + chord;
+      }
+      S footprint = true_m * static_cast<S>(kBeamWidth);
+      level = d.level_count - 1;
+      while (level > 0 && static_cast<S>(1 << (level * 2)) * res > footprint) --level;
+      dx = lsh_tangent(g.a1, g.a2, g.a3, g.a4, delta);
+      dy = lsh_tangent(g.b1, g.b2, g.b3, g.b4, delta);
+      inv_x = lsh_abs(dx) > S(1e-8) ? S(1) / dx : S(1e30);
+      inv_y = lsh_abs(dy) > S(1e-8) ? S(1) / dy : S(1e30);
+      ix = g.bx + static_cast<int>(lsh_floor(px));
+      iy = g.by + static_cast<int>(lsh_floor(py));
+      need_position = false;
+    }
+    const int shift = level * 2;
+    const int scale = 1 << shift;
+    const int cx = ix >> shift, cy = iy >> shift;
+    const int w = d.level_width[level], h = d.level_height[level];
+    if (!(cx < w && cy < h)) break;  // leaves the level: the ray ends
+    if (kCount) ++cnt.steps;
+    // hierarchy_bilinear_bound (cuda_backend.py:47-60): 2x2 cells clipped to the level
+    const float* base = level == 0 ? d.level0 : d.mips + d.level_offset[level];
+    const int cx1 = cx + 1 < w ? cx + 1 : w - 1;
+    const int cy1 = cy + 1 < h ? cy + 1 : h - 1;
+    const float* r0 = base + static_cast<int64_t>(cy) * w;
+    const float* r1 = base + static_cast<int64_t>(cy1) * w;
+    const float v00 = r0[cx], v10 = r0[cx1], v01 = r1[cx], v11 = r1[cx1];
+    const bool has_x = cx + 1 < w, has_y = cy + 1 < h;
+    const bool ok00 = lsh_valid(v00), ok10 = has_x && lsh_valid(v10);
+    const bool ok01 = has_y && lsh_valid(v01), ok11 = has_x && has_y && lsh_valid(v11);
+    float maxh = -32000.0f;
+    if (ok00 && v00 > maxh) maxh = v00;
+    if (ok10 && v10 > maxh) maxh = v10;
+    if (ok01 && v01 > maxh) maxh = v01;
+    if (ok11 && v11 > maxh) maxh = v11;
+    // cell exit along the local tangent, in km (cuda_backend.py:733-759)
+    const S ex0 = static_cast<S>(cx * scale - g.bx) - px;
+    const S ey0 = static_cast<S>(cy * scale - g.by) - py;
+    const S t1 = ex0 * inv_x, t2 = (ex0 + static_cast<S>(scale)) * inv_x;
+    const S t3 = ey0 * inv_y, t4 = (ey0 + static_cast<S>(scale)) * inv_y;
+    const S exit_distance = lsh_min(lsh_max(t1, t2), lsh_max(t3, t4));
+    const S fallback = ((static_cast<S>(scale) * res) * S(0.5)) * S(0.001);
+    const S to_exit = exit_distance > S(0) ? exit_distance : fallback;
+    const S skip = to_exit > S(0) ? to_exit + S(0.0001) : fallback;
+    if (maxh < -20000.0f) {  // nodata block
+      if (kCount) ++cnt.tests;
+      trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx), float(py + g.by),
+                maxh, 0.f, 0.f, float(skip), 2);
+      s.add(skip);
+      need_position = true;
+      continue;
+    }
+    const V true_near = static_cast<V>(lsh_max(true_m - static_cast<S>(scale) * res, S(1)));
+    const V hmax = static_cast<V>(maxh) - oz;
+    const V possible = near ? hmax / true_near
+                            : lsh_sphere_slope(hmax, oz, true_near, radius, two_observer_radius);
+    if (possible <= current) {  // cull
+      if (kCount) ++cnt.tests;
+      trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx), float(py + g.by),
+                maxh, 0.f, float(possible), float(skip), 1);
+      s.add(skip);
+      need_position = true;
+      continue;
+    }
+    if (level == 0) {
+      // bilinear (cuda_backend.py:107-132) of h - oz on the same 2x2 taps
+      V slope = V(-1e30);
+      if (ok00 && ok10 && ok01 && ok11) {
+        S sx_frac = px - lsh_floor(px), sy_frac = py - lsh_floor(py);
+        if (ix == d.width - 2 && sx_frac > clamp_tx) sx_frac = clamp_tx;
+        if (iy == d.height - 2 && sy_frac > clamp_ty) sy_frac = clamp_ty;
+        const V tx = static_cast<V>(sx_frac), ty = static_cast<V>(sy_frac);
+        const V d00 = V(v00) - oz, d10 = V(v10) - oz, d01 = V(v01) - oz, d11 = V(v11) - oz;
+        const V top = d00 + tx * (d10 - d00);
+        const V bottom = d01 + tx * (d11 - d01);
+        const V hd = top + ty * (bottom - top);
+        const V tm = static_cast<V>(true_m);
+        slope = near ? hd / tm : lsh_sphere_slope(hd, oz, tm, radius, two_observer_radius);
+        if (slope > current) {
+          current = slope;
+          best = (pass << 24) | sample_index;
+        }
+      }
+      if (kCount) ++cnt.samples;
+      ++sample_index;
+      const S mag = lsh_sqrt(dx * dx + dy * dy);
+      const S pixel_step = mag > S(1e-6) ? S(1) / mag : S(0.0005);
+      const S margin = static_cast<S>(current - slope);
+      const S margin_step = margin > S(0) ? ((margin * true_m) * S(1.732)) / S(1000) : S(0);
+      const S angular_step = (true_m * S(0.00151)) / S(1000);
+      S advance = lsh_max(pixel_step, lsh_min(margin_step, angular_step));
+      if (near) advance = advance * S(0.25);
+      const S floor_step = (pass == 0 && true_m >= S(100)) ? far_step : min_step;
+      advance = lsh_max(advance, floor_step);
+      if (kCap) advance = lsh_min(advance, to_exit > S(0) ? to_exit + S(0.000001) : fallback);
+      trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx), float(py + g.by),
+                maxh, 0.f, float(slope), float(advance), 4);
+      s.add(advance);
+      need_position = true;
+      continue;
+    }
+    if (kCount) ++cnt.tests;
+    trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx), float(py + g.by),
+              maxh, 0.f, float(possible), 0.f, 0);
+    --level;
+  }
+}
+
+LSH_HD int lsh_clamp_center(int requested, int dem_size, int sub) {
+  int half = sub / 2;
+  int maximum = dem_size - half;
+  if (maximum < half) maximum = half;
+  return requested < half ? half : (requested > maximum ? maximum : requested);
+}
+
+// bilinear (cuda_backend.py:107-132) of a DEM at an integer observer pixel.
+LSH_HD float lsh_observer_terrain(const float* e, int w, int h, float column, float row) {
+  column = __builtin_fminf(__builtin_fmaxf(column, 0.0f), static_cast<float>(w) - 1.0001f);
+  row = __builtin_fminf(__builtin_fmaxf(row, 0.0f), static_cast<float>(h) - 1.0001f);
+  int x0 = static_cast<int>(__builtin_floorf(column)), y0 = static_cast<int>(__builtin_floorf(row));
+  int x1 = x0 + 1 < w ? x0 + 1 : w - 1, y1 = y0 + 1 < h ? y0 + 1 : h - 1;
+  float q00 = e[static_cast<int64_t>(y0) * w + x0], q10 = e[static_cast<int64_t>(y0) * w + x1];
+  float q01 = e[static_cast<int64_t>(y1) * w + x0], q11 = e[static_cast<int64_t>(y1) * w + x1];
+  if (!(lsh_valid(q00) && lsh_valid(q10) && lsh_valid(q01) && lsh_valid(q11))) return -32000.0f;
+  float tx = column - static_cast<float>(x0), ty = row - static_cast<float>(y0);
+  float top = q00 + tx * (q10 - q00);
+  float bottom = q01 + tx * (q11 - q01);
+  return top + ty * (bottom - top);
+}
+
+// Per-pixel subpatch selection (cuda_backend.py:1040-1095).
+struct PixelSelection {
+  int32_t i00, i10, i01, i11;
+  float tx, ty;
+  int32_t cl, cr, ct, cb;  // clamped centers relative to the tile origin
+};
+
+LSH_HD PixelSelection lsh_select(int column, int row, int tile_column, int tile_row,
+                                 int tile_width, int sub, int primary_w, int primary_h) {
+  PixelSelection s;
+  const int count = tile_width / sub + 2;
+  float gx = static_cast<float>(column - sub / 2) / static_cast<float>(sub) + 1.0f;
+  float gy = static_cast<float>(row - sub / 2) / static_cast<float>(sub) + 1.0f;
+  int left = static_cast<int>(gx), top = static_cast<int>(gy);
+  s.tx = gx - static_cast<float>(left);
+  s.ty = gy - static_cast<float>(top);
+  if (left < 0) { left = 0; s.tx = 0.0f; }
+  if (top < 0) { top = 0; s.ty = 0.0f; }
+  if (left > count - 2) { left = count - 2; s.tx = 1.0f; }
+  if (top > count - 2) { top = count - 2; s.ty = 1.0f; }
+  s.i00 = top * count + left;
+  s.i10 = s.i00 + 1;
+  s.i01 = (top + 1) * count + left;
+  s.i11 = s.i01 + 1;
+  const int rl = tile_column + (left - 1) * sub + sub / 2;
+  const int rt = tile_row + (top - 1) * sub + sub / 2;
+  s.cl = lsh_clamp_center(rl, primary_w, sub) - tile_column;
+  s.cr = lsh_clamp_center(rl + sub, primary_w, sub) - tile_column;
+  s.ct = lsh_clamp_center(rt, primary_h, sub) - tile_row;
+  s.cb = lsh_clamp_center(rt + sub, primary_h, sub) - tile_row;
+  return s;
+}
+
+// interpolate_subpatch_field (cuda_backend.py:965-987) of one pass, in precision S.
+// p00..p11 point at the 18 fields of the four corner segments; start
+// coordinates are taken relative to the integer base floor(p00[0]),
+// floor(p00[1]) before the shifts and the bilinear blend (exact: the corners
+// lie within a few pixels).
+template <class S>
+LSH_HD PassSegment<S> lsh_interpolate(const float* p00, const float* p10, const float* p01,
+                                      const float* p11, const PixelSelection& sel, int column,
+                                      int row, float ratio) {
+  PassSegment<S> g;
+  const S tx = sel.tx, ty = sel.ty;
+  g.bx = static_cast<int32_t>(__builtin_floorf(p00[0]));
+  g.by = static_cast<int32_t>(__builtin_floorf(p00[1]));
+  const S fbx = static_cast<S>(g.bx), fby = static_cast<S>(g.by);
+  const S r = static_cast<S>(ratio);
+  const S sxl = static_cast<S>(column - sel.cl) * r;
+  const S sxr = static_cast<S>(column - sel.cr) * r;
+  const S syt = static_cast<S>(row - sel.ct) * r;
+  const S syb = static_cast<S>(row - sel.cb) * r;
+  auto fma = [](S a, S b, S c) {
+    return sizeof(S) == 4 ? static_cast<S>(__builtin_fmaf(float(a), float(b), float(c)))
+                          : static_cast<S>(__builtin_fma(double(a), double(b), double(c)));
+  };
+  auto blend = [&](S v00, S v10, S v01, S v11) {
+    S upper = fma(v10 - v00, tx, v00);
+    S lower = fma(v11 - v01, tx, v01);
+    return fma(lower - upper, ty, upper);
+  };
+  g.sx = blend((S(p00[0]) - fbx) + sxl, (S(p10[0]) - fbx) + sxr, (S(p01[0]) - fbx) + sxl,
+               (S(p11[0]) - fbx) + sxr);
+  g.sy = blend((S(p00[1]) - fby) + syt, (S(p10[1]) - fby) + syt, (S(p01[1]) - fby) + syb,
+               (S(p11[1]) - fby) + syb);
+#define LSH_FIELD(f) blend(S(p00[f]), S(p10[f]), S(p01[f]), S(p11[f]))
+  g.a1 = LSH_FIELD(4); g.a2 = LSH_FIELD(5); g.a3 = LSH_FIELD(6); g.a4 = LSH_FIELD(7);
+  g.b1 = LSH_FIELD(8); g.b2 = LSH_FIELD(9); g.b3 = LSH_FIELD(10); g.b4 = LSH_FIELD(11);
+  g.s_start = LSH_FIELD(12); g.s_end = LSH_FIELD(13); g.s_chord = LSH_FIELD(14);
+  g.c1 = LSH_FIELD(15); g.c2 = LSH_FIELD(16); g.c3 = LSH_FIELD(17);
+#undef LSH_FIELD
+  return g;
+}
+
+}  // namespace lsh

@@ -418,3 +418,20 @@ def cpu_threads() -> int:
         return len(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
         return os.cpu_count() or 1
+
+
+def trace_tile_ray(pyramids, segment_values, *, tile_column: int, tile_row: int, pixel: int,
+                   azimuth: int, tile_width: int = 128, subpatch_size: int = 8,
+                   observer_elevation_m: float = 0.0, capacity: int = 200000):
+    """Float64 trace rows of one (pixel, azimuth) ray of a tile over every pass."""
+    keep = _Keep()
+    segs = np.ascontiguousarray(segment_values, dtype=np.float32)
+    N, D = segs.shape[0], segs.shape[2]
+    arr = (LshoPyramid * D)(*[p.struct(keep) for p in pyramids])
+    rows = np.zeros((capacity, 12), dtype=np.float32)
+    result = _f64()
+    n = lib().lsho_trace_tile_ray(
+        arr, _i32(D), _vp(segs.ctypes.data), _i32(N), _i32(tile_column), _i32(tile_row),
+        _i32(tile_width), _i32(subpatch_size), _f64(observer_elevation_m), _i32(pixel),
+        _i32(azimuth), _vp(rows.ctypes.data), _i32(capacity), ctypes.byref(result))
+    return result.value, rows[: min(n, capacity)].copy()

@@ -1143,6 +1143,23 @@ int lsho_horizon_tile(const lsho_pyramid* pyrs, int32_t D, const float* segments
   return 0;
 }
 
+// Float64 trace of one (pixel, azimuth) ray of a tile over every pass (rows as in
+// lsho_march_ray, concatenated pass after pass). Returns the row count.
+int lsho_trace_tile_ray(const lsho_pyramid* pyrs, int32_t D, const float* segments, int32_t N,
+                        int32_t tile_column, int32_t tile_row, int32_t tile_width,
+                        int32_t subpatch, double obs_h, int32_t pixel, int32_t az,
+                        float* trace, int32_t trace_cap, double* result) {
+  int count = tile_width / subpatch + 2;
+  MarchOpts o{};
+  o.stable_slope = true;
+  o.trace = trace;
+  o.trace_cap = trace_cap;
+  *result = ray_all_passes<double>(pyrs, D, segments, N, count * count, az, pixel % tile_width,
+                                   pixel / tile_width, tile_column, tile_row, tile_width,
+                                   subpatch, obs_h, o);
+  return o.trace_count;
+}
+
 // Same as lsho_horizon_tile but returns float64 slopes (method mode only), so
 // parity figures are stated against the unrounded float64 result.
 int lsho_horizon_tile_f64(const lsho_pyramid* pyrs, int32_t D, const float* segments,
