@@ -1,0 +1,264 @@
+#!/usr/bin/env python3
+"""Full-site horizon + lighting benchmark on one MI355X.
+
+Workload (synthetic, seeded): a 2048x2048 float32 primary DEM at 5 m/px (fBm
+Hurst 0.8, 300 m RMS, 40 paraboloid craters, seed 1) and a 4096x4096 float32
+regional DEM at 60 m/px (seed 2), both centred on an 89.5 S site in south-polar
+stereographic; 256 horizon tiles of 128x128 px x 1440 azimuths over both DEM
+passes, then from those horizons the lightmap (finite-disk sun fraction bytes
+per time), the Earth-visibility fraction and the PSR mask for 125 Sun and 125
+Earth Moon-ME vectors at 6-hour steps over 31 days.
+
+A step is one batch of tiles (``--tiles-per-step``, default 32): FP64 segment
+generation, the fused two-pass march into HBM (degrees), and the streamed
+lighting pass over the batch. The default 8 timed steps cover the 256 tiles
+once. DEMs and vectors are resident in HBM before the timed region.
+
+Prints one JSON line (see the fields below).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "horizon tiles/s + lightmap px·times/s, 1 MI355X; batched vs plain at ≤0.01° err"
+HBM_PEAK_GBS = 8000.0
+AZIMUTHS = 1440
+PIXELS = 128 * 128
+
+
+def log(*args):
+    print(*args, file=sys.stderr, flush=True)
+
+
+def parse_args():
+    p = argparse.ArgumentParser(description=__doc__)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=8)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--tiles-per-step", type=int, default=32)
+    p.add_argument("--cpu-sample-pixels", type=int, default=256,
+                   help="pixels of one tile the CPU baseline marches (0 disables it)")
+    p.add_argument("--counter-tiles", type=int, default=4,
+                   help="tiles re-run with traversal counters for the roofline")
+    return p.parse_args()
+
+
+def main() -> int:
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(tiles shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        return 2
+    from lunarscout_amd import synthetic
+    from lunarscout_amd._lib import DeviceBuffer, Event, HORIZON_COUNTERS, HORIZON_DEGREES
+    from lunarscout_amd.hip_backend import HipSession
+    from lunarscout_amd.lighting import HipLightingSession, reduce_sun_vectors_for_psr
+
+    t0 = time.time()
+    dems = synthetic.site_dems(synthetic.FULL_SITE)
+    primary = dems[0]
+    sun = synthetic.sun_vectors(primary, 125, 6.0, seed=3)
+    earth = synthetic.earth_vectors(125, 6.0, seed=4)
+    psr_vectors, _ = reduce_sun_vectors_for_psr(primary, sun)
+    log(f"inputs generated in {time.time() - t0:.1f}s; PSR vectors {len(psr_vectors)}")
+
+    session = HipSession(0)
+    log("device:", session.device_name())
+    for d in dems:
+        session.dem_id(d)
+    session.synchronize()
+    lighting = HipLightingSession(session)
+    sun_d = DeviceBuffer.from_host(session, sun)
+    earth_d = DeviceBuffer.from_host(session, earth)
+    psr_d = DeviceBuffer.from_host(session, psr_vectors)
+
+    tiles = [(x, y) for y in range(0, primary.height, 128) for x in range(0, primary.width, 128)]
+    per_step = args.tiles_per_step
+    horizons = DeviceBuffer.empty(session, (per_step, PIXELS, AZIMUTHS), np.float32)
+    segments = None
+    outputs = None
+    stats = {"march_ms": 0.0, "march_launches": 0, "segments_ms": 0.0, "lighting_ms": 0.0,
+             "lighting_launches": 0, "tiles": 0}
+
+    def step(index: int, record: bool):
+        nonlocal segments, outputs
+        batch = [tiles[(index * per_step + i) % len(tiles)] for i in range(per_step)]
+        events = []
+        for slot, (c, r) in enumerate(batch):
+            e0 = Event(session).record()
+            segments = session.segments_device(dems, tile_column=c, tile_row=r, out=segments)
+            e1 = Event(session).record()
+            view = _slot_view(horizons, slot)
+            session.march_device(dems, segments, tile_column=c, tile_row=r,
+                                 flags=HORIZON_DEGREES, output=view)
+            e2 = Event(session).record()
+            events.append((e0, e1, e2))
+        l0 = Event(session).record()
+        outputs = lighting.lighting_tiles(primary, batch, horizons, sun=sun_d, earth=earth_d,
+                                          psr_vectors=psr_d, sun_bytes=True, sun_mean=True,
+                                          earth_visible=True, psr=True, outputs=outputs)
+        l1 = Event(session).record()
+        if record:
+            for e0, e1, e2 in events:
+                stats["segments_ms"] += e0.elapsed_ms(e1)
+                stats["march_ms"] += e1.elapsed_ms(e2)
+                stats["march_launches"] += 1
+            stats["lighting_ms"] += l0.elapsed_ms(l1)
+            stats["lighting_launches"] += 1
+            stats["tiles"] += len(batch)
+
+    for w in range(args.warmup):
+        step(w, record=False)
+    session.synchronize()
+    log(f"warmup done ({args.warmup} steps)")
+    start = time.perf_counter()
+    for k in range(args.steps):
+        step(args.warmup + k, record=True)
+        log(f"step {k + 1}/{args.steps} done")
+    session.synchronize()
+    elapsed = time.perf_counter() - start
+
+    tiles_per_s = stats["tiles"] / elapsed
+    march_ms = stats["march_ms"] / max(stats["march_launches"], 1)
+    lighting_ms = stats["lighting_ms"] / max(stats["lighting_launches"], 1)
+    light_px_times = per_step * PIXELS * (len(sun) + len(earth) + len(psr_vectors))
+
+    # traversal counters on a spread of tiles -> algorithmic bytes per tile
+    counters = DeviceBuffer.empty(session, (4,), np.int64)
+    samples = tests = 0
+    pick = [tiles[i] for i in np.linspace(0, len(tiles) - 1, args.counter_tiles).astype(int)]
+    for c, r in pick:
+        counters.zero()
+        segments = session.segments_device(dems, tile_column=c, tile_row=r, out=segments)
+        session.march_device(dems, segments, tile_column=c, tile_row=r, flags=HORIZON_COUNTERS,
+                             output=_slot_view(horizons, 0), counters=counters)
+        v = counters.to_host()
+        samples += int(v[0])
+        tests += int(v[1])
+    rays = PIXELS * AZIMUTHS
+    S, T = samples / len(pick), tests / len(pick)
+    passes = len(dems)
+    bytes_alg = 16.0 * (S + T) + 256.0 * rays * passes + 4.0 * rays
+    achieved = bytes_alg / (march_ms / 1000.0) / 1e9
+    traffic = _traffic_from_profile()
+
+    result = {
+        "metric": METRIC,
+        "value": round(tiles_per_s, 4),
+        "unit": "tiles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1000.0, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (heights, slopes) + f64 (ray geometry)",
+        "data": "synthetic (seeded fBm + craters DEMs, synthetic Moon-ME vectors)",
+        "config": {
+            "workload": "full site, plain HIP path: 2048^2 @5 m primary + 4096^2 @60 m regional, "
+                        "256 tiles x 1440 az, lighting + Earth + PSR from 125 vectors",
+            "tiles_per_step": per_step,
+            "seeds": {"primary": 1, "regional": 2, "sun": 3, "earth": 4},
+        },
+        "horizon": {
+            "march_ms_per_tile": round(march_ms, 3),
+            "segments_ms_per_tile": round(stats["segments_ms"] / max(stats["march_launches"], 1), 3),
+            "level0_samples_per_tile": S,
+            "bound_tests_per_tile": T,
+        },
+        "lighting": {
+            "value": round(light_px_times / (lighting_ms / 1000.0), 1),
+            "unit": "px·times/s (sun + earth + PSR vectors)",
+            "kernel_ms_per_batch": round(lighting_ms, 3),
+            "times": {"sun": len(sun), "earth": len(earth), "psr": len(psr_vectors)},
+        },
+        "roofline": {
+            "kernel": "march_kernel",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_alg_per_launch": bytes_alg,
+        },
+    }
+    if args.cpu_sample_pixels > 0:
+        result["cpu_baseline"] = _cpu_baseline(dems, sun, args.cpu_sample_pixels)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def _slot_view(buffer, slot):
+    """A DeviceBuffer-like view of one tile slot of a [slots, pixels, N] buffer."""
+    from lunarscout_amd._lib import DeviceBuffer
+
+    view = DeviceBuffer.__new__(DeviceBuffer)
+    view.session = buffer.session
+    view.dtype = buffer.dtype
+    view.shape = buffer.shape[1:]
+    view.nbytes = int(np.prod(view.shape)) * view.dtype.itemsize
+    view.ptr = buffer.ptr + slot * view.nbytes
+    view.free = lambda: None  # the parent owns the memory
+    return view
+
+
+def _traffic_from_profile():
+    """HBM bytes per march launch from the committed rocprofv3 PMC summary, if present."""
+    path = ROOT / "profiles" / "march_full_site_pmc.json"
+    if not path.exists():
+        return None
+    try:
+        return json.loads(path.read_text()).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def _cpu_baseline(dems, sun, sample_pixels):
+    """The C++/OpenMP oracle (float32 reference arithmetic) on a bounded sample."""
+    import oracle
+
+    cores = oracle.cpu_threads()
+    pyramids = [oracle.OraclePyramid(d) for d in dems]
+    tile = (1024, 1024)
+    t0 = time.perf_counter()
+    segs = oracle.segments(dems, tile_column=tile[0], tile_row=tile[1], tile_width=128,
+                           azimuth_count=AZIMUTHS, threads=cores)
+    t_seg = time.perf_counter() - t0
+    pixels = np.linspace(0, PIXELS - 1, sample_pixels).astype(np.int32)
+    t0 = time.perf_counter()
+    oracle.horizon_tile(pyramids, segs, tile_column=tile[0], tile_row=tile[1], pixels=pixels,
+                        mode=oracle.FLOAT32 | oracle.REFERENCE_SLOPE, threads=cores)
+    t_march = time.perf_counter() - t0
+    per_tile = t_seg + t_march * PIXELS / sample_pixels
+    horizons = np.zeros((128, 128, AZIMUTHS), np.float32)
+    t0 = time.perf_counter()
+    oracle.lighting_patch(dems[0], horizons, sun, tile_y=tile[1], tile_x=tile[0],
+                          float32_slices=True, threads=cores)
+    t_light = time.perf_counter() - t0
+    return {
+        "value": round(1.0 / per_tile, 6),
+        "unit": "tiles/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"tile {tile}: full FP64 segment generation ({t_seg:.2f}s) + float32 march of "
+                  f"{sample_pixels} of 16384 px x 1440 az x 2 passes ({t_march:.2f}s), scaled "
+                  f"to one tile; lighting: one tile x {len(sun)} Sun times ({t_light:.2f}s)",
+        "lighting_px_times_per_s": round(PIXELS * len(sun) / t_light, 1),
+    }
+
+
+if __name__ == "__main__":
+    sys.exit(main())

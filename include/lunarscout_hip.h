@@ -36,6 +36,8 @@ extern "C" {
 #define LSH_HORIZON_DEGREES 1   /* write float32 degrees (contract.py:362-371) instead of slopes */
 #define LSH_HORIZON_CAP 2       /* diagnostic: cap level-0 advances at the cell exit (hierarchy.py:258-262) */
 #define LSH_HORIZON_COUNTERS 4  /* accumulate traversal counters (hierarchy.py:40-52) */
+#define LSH_HORIZON_F32_VALUES 8 /* heights, bounds and slopes in float32 (faster; float32 ties
+                                    let rare rays drift from the float64 method) */
 
 /* lsh_lighting_tiles flags */
 #define LSH_LIGHT_SUN_BYTES 1     /* per-time uint8 sun fraction bytes */
@@ -81,6 +83,15 @@ int lsh_copy_to_host(lsh_session* session, void* host_dst, const void* device_sr
 int lsh_memset_device(lsh_session* session, void* device_dst, int value, size_t bytes);
 int lsh_synchronize(lsh_session* session);
 
+/* Timing events on the session stream (hipEvent): create, record at the
+ * stream's current position, and read the milliseconds between two recorded
+ * events (waits for the later one). No reference counterpart: the reference
+ * times passes with cuda.event (cuda_backend.py:1491-1536). */
+int lsh_event_create(lsh_session* session, void** event);
+int lsh_event_record(lsh_session* session, void* event);
+int lsh_event_elapsed_ms(void* start, void* stop, float* milliseconds);
+int lsh_event_destroy(void* event);
+
 /* Register a DEM on the device and build its factor-four max pyramid there
  * (CudaSession.build_max_pyramid, cuda_backend.py:1260-1281, and the resident
  * pyramids of _prepare_production_pyramids, cuda_backend.py:1231-1258).
@@ -122,7 +133,8 @@ int lsh_segments_tile(lsh_session* session, const int32_t* dem_ids, int32_t dem_
  * cuda_backend.py:1007-1134). segments (device): float32[N][C][D][18];
  * output (device): float32[tile_height*tile_width][N], pixel-major; pixels
  * outside the primary DEM are -inf. counters (device, nullable, with
- * LSH_HORIZON_COUNTERS): int64[3] += (level-0 samples, bound tests, steps). */
+ * LSH_HORIZON_COUNTERS): int64[4] += (level-0 samples, bound tests, steps,
+ * 64 x the longest lane's steps per wavefront). */
 int lsh_horizon_tile(lsh_session* session, const int32_t* dem_ids, int32_t dem_count,
                      const float* segments, int32_t azimuth_count, int32_t tile_column,
                      int32_t tile_row, int32_t tile_width, int32_t tile_height,
@@ -135,6 +147,44 @@ int lsh_horizon_tile_full(lsh_session* session, const int32_t* dem_ids, int32_t 
                           int32_t tile_column, int32_t tile_row, int32_t azimuth_count,
                           double maximum_distance_m, float observer_elevation_m,
                           int32_t flags, float* output, int64_t* counters);
+
+/* Lighting of stored horizon tiles in one streamed pass (config of the
+ * reference's LightmapCudaSession.iter_patch_tiles / iter_patch_fraction_tiles /
+ * iter_patch_margin_tiles, lightmap_cuda.py:187-276, and
+ * PsrCudaSession.compute_patch, psr_cuda.py:238-375).
+ * tiles (host): tile origins in the DEM; horizons (device): float32 degrees,
+ * [tile][16384][N] (pixel-major, the tile file layout) or, with
+ * azimuth_major = 1, [tile][N][16384]. sun / earth / psr_vectors (device):
+ * float64 Moon-ME positions [count][3] (nullable when the count is 0).
+ * sun_half_angle_deg: the disk half angle of the 16-slice sun (reference 0.27);
+ * psr_diameter_deg: the PSR upper-limb diameter (reference 0.545).
+ * Outputs (device, each nullable, selected by flags LSH_LIGHT_*):
+ *   sun_bytes uint8 [tile][sun_count][16384] = uint8(255 * fraction)
+ *   sun_fraction float [tile][sun_count][16384]; sun_mean float [tile][16384]
+ *   earth_margin float [tile][earth_count][16384]  (elevation - horizon, deg)
+ *   earth_visible float [tile][16384]  (fraction of times with margin >= 0)
+ *   psr uint8 [tile][16384] (255 = PSR); psr_margin float [tile][16384]
+ * Pixels outside the DEM get 0 / -inf. */
+int lsh_lighting_tiles(lsh_session* session, int32_t dem_id, const lsh_tile* tiles,
+                       int32_t tile_count, const float* horizons, int32_t azimuth_count,
+                       int32_t azimuth_major, const double* sun, int32_t sun_count,
+                       const double* earth, int32_t earth_count, const double* psr_vectors,
+                       int32_t psr_count, double sun_half_angle_deg, double psr_diameter_deg,
+                       int32_t flags, uint8_t* sun_bytes, float* sun_fraction, float* sun_mean,
+                       float* earth_margin, float* earth_visible, uint8_t* psr,
+                       float* psr_margin);
+
+/* Five-viewpoint, per-azimuth-bin maximum-elevation reduction of Sun vectors
+ * for PSR (reduce_sun_vectors_for_psr, psr.py:61-91). Host-only float64.
+ * elevation: host float32[height][width]; vectors float64[count][3];
+ * indices: int64[5 * azimuth_count] capacity, written in first-seen order. */
+int lsh_psr_reduce(const float* elevation, int32_t width, int32_t height,
+                   const double* geo_transform, const double* projection, const double* vectors,
+                   int32_t count, int32_t azimuth_count, int64_t* indices, int32_t* kept);
+
+/* Sun-disk slice table of the lightmap (lightmap.py:15-29) for a half angle:
+ * half_circle float32[16] and max_photons. */
+int lsh_sun_disk(double sun_half_angle_deg, float* half_circle, float* max_photons);
 
 #ifdef __cplusplus
 }

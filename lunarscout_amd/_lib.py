@@ -26,6 +26,7 @@ LSH_ENOMEM = -4
 HORIZON_DEGREES = 1
 HORIZON_CAP = 2
 HORIZON_COUNTERS = 4
+HORIZON_F32_VALUES = 8
 
 LIGHT_SUN_BYTES = 1
 LIGHT_SUN_FRACTION = 2
@@ -69,6 +70,14 @@ SIGNATURES = {
     "lsh_horizon_tile": [_vp, _vp, _i32, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _f32, _i32,
                          _vp, _vp],
     "lsh_horizon_tile_full": [_vp, _vp, _i32, _i32, _i32, _i32, _f64, _f32, _i32, _vp, _vp],
+    "lsh_lighting_tiles": [_vp, _i32, _vp, _i32, _vp, _i32, _i32, _vp, _i32, _vp, _i32, _vp, _i32,
+                           _f64, _f64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "lsh_psr_reduce": [_vp, _i32, _i32, _vp, _vp, _vp, _i32, _i32, _vp, ctypes.POINTER(_i32)],
+    "lsh_sun_disk": [_f64, _vp, _vp],
+    "lsh_event_create": [_vp, ctypes.POINTER(_vp)],
+    "lsh_event_record": [_vp, _vp],
+    "lsh_event_elapsed_ms": [_vp, _vp, ctypes.POINTER(_f32)],
+    "lsh_event_destroy": [_vp],
 }
 
 
@@ -180,6 +189,33 @@ class DeviceBuffer:
     def __del__(self):  # pragma: no cover - best effort
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class Event:
+    """A hipEvent recorded on a session's stream."""
+
+    def __init__(self, session: "Session"):
+        self.session = session
+        handle = _vp()
+        check(load_library().lsh_event_create(session.handle, ctypes.byref(handle)),
+              "lsh_event_create")
+        self.handle = handle.value
+
+    def record(self) -> "Event":
+        check(load_library().lsh_event_record(self.session.handle, self.handle), "lsh_event_record")
+        return self
+
+    def elapsed_ms(self, stop: "Event") -> float:
+        ms = _f32()
+        check(load_library().lsh_event_elapsed_ms(self.handle, stop.handle, ctypes.byref(ms)),
+              "lsh_event_elapsed_ms")
+        return float(ms.value)
+
+    def __del__(self):  # pragma: no cover
+        try:
+            load_library().lsh_event_destroy(self.handle)
         except Exception:
             pass
 

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "lsh_internal.h"
 
@@ -73,6 +74,37 @@ int select_device(Session* s) {
 }
 
 }  // namespace
+
+namespace lsh {
+
+void sun_disk(double half_angle_deg, float* half_circle, float* max_photons) {
+  float angle = static_cast<float>(half_angle_deg);
+  float sum = 0.0f;
+  for (int i = 0; i < 16; ++i) {
+    half_circle[i] = static_cast<float>(std::sqrt(64.0 - (7.5 - i) * (7.5 - i)) / 8.0 *
+                                        static_cast<double>(angle));
+    sum = sum + half_circle[i];
+  }
+  *max_photons = 2.0f * sum;
+}
+
+int upload_tiles(Session* s, const lsh_tile* tiles, int32_t count, const lsh_tile** device_tiles) {
+  if (s->tile_scratch_count < count) {
+    LSH_CHECK_HIP(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    if (s->tile_scratch) hipFree(s->tile_scratch);
+    s->tile_scratch = nullptr;
+    s->tile_scratch_count = 0;
+    LSH_CHECK_HIP(hipMalloc(&s->tile_scratch, sizeof(lsh_tile) * count), "hipMalloc tile list");
+    s->tile_scratch_count = count;
+  }
+  LSH_CHECK_HIP(hipMemcpyAsync(s->tile_scratch, tiles, sizeof(lsh_tile) * count,
+                               hipMemcpyHostToDevice, s->stream),
+                "copy tile list");
+  *device_tiles = s->tile_scratch;
+  return LSH_OK;
+}
+
+}  // namespace lsh
 
 extern "C" {
 
@@ -148,6 +180,7 @@ int lsh_session_destroy(lsh_session* session) {
     }
   }
   if (s->segment_scratch) hipFree(s->segment_scratch);
+  if (s->tile_scratch) hipFree(s->tile_scratch);
   hipStreamDestroy(s->stream);
   delete s;
   return LSH_OK;
@@ -201,6 +234,37 @@ int lsh_synchronize(lsh_session* session) {
   LSH_REQUIRE(s, "invalid session");
   if (int r = select_device(s)) return r;
   LSH_CHECK_HIP(hipStreamSynchronize(s->stream), "kernel execution");
+  return LSH_OK;
+}
+
+int lsh_event_create(lsh_session* session, void** event) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(s && event, "invalid arguments");
+  if (int r = select_device(s)) return r;
+  hipEvent_t e;
+  LSH_CHECK_HIP(hipEventCreate(&e), "hipEventCreate");
+  *event = e;
+  return LSH_OK;
+}
+
+int lsh_event_record(lsh_session* session, void* event) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(s && event, "invalid arguments");
+  LSH_CHECK_HIP(hipEventRecord(static_cast<hipEvent_t>(event), s->stream), "hipEventRecord");
+  return LSH_OK;
+}
+
+int lsh_event_elapsed_ms(void* start, void* stop, float* milliseconds) {
+  LSH_REQUIRE(start && stop && milliseconds, "invalid arguments");
+  LSH_CHECK_HIP(hipEventSynchronize(static_cast<hipEvent_t>(stop)), "hipEventSynchronize");
+  LSH_CHECK_HIP(hipEventElapsedTime(milliseconds, static_cast<hipEvent_t>(start),
+                                    static_cast<hipEvent_t>(stop)),
+                "hipEventElapsedTime");
+  return LSH_OK;
+}
+
+int lsh_event_destroy(void* event) {
+  if (event) hipEventDestroy(static_cast<hipEvent_t>(event));
   return LSH_OK;
 }
 
@@ -418,6 +482,139 @@ int lsh_horizon_tile_full(lsh_session* session, const int32_t* dem_ids, int32_t 
   return lsh_horizon_tile(session, dem_ids, dem_count, s->segment_scratch, azimuth_count,
                           tile_column, tile_row, 128, 128, 8, observer_elevation_m, flags,
                           output, counters);
+}
+
+int lsh_sun_disk(double sun_half_angle_deg, float* half_circle, float* max_photons) {
+  LSH_REQUIRE(half_circle && max_photons, "invalid arguments");
+  LSH_REQUIRE(std::isfinite(sun_half_angle_deg) && sun_half_angle_deg > 0.0,
+              "sun half angle must be positive");
+  lsh::sun_disk(sun_half_angle_deg, half_circle, max_photons);
+  return LSH_OK;
+}
+
+int lsh_lighting_tiles(lsh_session* session, int32_t dem_id, const lsh_tile* tiles,
+                       int32_t tile_count, const float* horizons, int32_t azimuth_count,
+                       int32_t azimuth_major, const double* sun, int32_t sun_count,
+                       const double* earth, int32_t earth_count, const double* psr_vectors,
+                       int32_t psr_count, double sun_half_angle_deg, double psr_diameter_deg,
+                       int32_t flags, uint8_t* sun_bytes, float* sun_fraction, float* sun_mean,
+                       float* earth_margin, float* earth_visible, uint8_t* psr,
+                       float* psr_margin) {
+  Session* s = reinterpret_cast<Session*>(session);
+  Dem* d = find_dem(s, dem_id);
+  LSH_REQUIRE(d, "unknown DEM id");
+  LSH_REQUIRE(tiles && tile_count > 0 && tile_count <= 65535, "tile_count must be 1..65535");
+  LSH_REQUIRE(horizons && azimuth_count >= 2, "horizons and azimuth_count >= 2 are required");
+  LSH_REQUIRE(sun_count >= 0 && earth_count >= 0 && psr_count >= 0, "counts must be >= 0");
+  LSH_REQUIRE(!sun_count || sun, "sun vectors missing");
+  LSH_REQUIRE(!earth_count || earth, "earth vectors missing");
+  LSH_REQUIRE(!psr_count || psr_vectors, "PSR vectors missing");
+  LSH_REQUIRE(std::isfinite(sun_half_angle_deg) && sun_half_angle_deg > 0.0 &&
+                  std::isfinite(psr_diameter_deg) && psr_diameter_deg > 0.0,
+              "disk sizes must be positive");
+  for (int i = 0; i < tile_count; ++i)
+    LSH_REQUIRE(tiles[i].tile_column >= 0 && tiles[i].tile_row >= 0 &&
+                    tiles[i].tile_column < d->width && tiles[i].tile_row < d->height,
+                "tile origin lies outside the DEM");
+  lsh::LightArgs a{};
+  a.elevation = d->level0;
+  a.dem_width = d->width;
+  a.dem_height = d->height;
+  std::memcpy(a.geo_transform, d->geo_transform, sizeof(a.geo_transform));
+  std::memcpy(a.projection, d->projection, sizeof(a.projection));
+  a.tile_count = tile_count;
+  a.horizons = horizons;
+  a.N = azimuth_count;
+  a.azimuth_major = azimuth_major ? 1 : 0;
+  a.sun = sun;
+  a.sun_count = sun_count;
+  a.earth = earth;
+  a.earth_count = earth_count;
+  a.psr_vectors = psr_vectors;
+  a.psr_count = psr_count;
+  lsh::sun_disk(sun_half_angle_deg, a.half_circle, &a.max_photons);
+  a.sun_half_angle = static_cast<float>(sun_half_angle_deg);
+  a.psr_threshold = -static_cast<double>(static_cast<float>(psr_diameter_deg)) / 2.0;
+  a.sun_bytes = (flags & LSH_LIGHT_SUN_BYTES) ? sun_bytes : nullptr;
+  a.sun_fraction = (flags & LSH_LIGHT_SUN_FRACTION) ? sun_fraction : nullptr;
+  a.sun_mean = (flags & LSH_LIGHT_SUN_MEAN) ? sun_mean : nullptr;
+  a.earth_margin = (flags & LSH_LIGHT_EARTH_MARGIN) ? earth_margin : nullptr;
+  a.earth_visible = (flags & LSH_LIGHT_EARTH_VISIBLE) ? earth_visible : nullptr;
+  a.psr = (flags & LSH_LIGHT_PSR) ? psr : nullptr;
+  a.psr_margin = (flags & LSH_LIGHT_PSR_MARGIN) ? psr_margin : nullptr;
+  LSH_REQUIRE(!(flags & LSH_LIGHT_SUN_BYTES) || sun_bytes, "sun_bytes output missing");
+  LSH_REQUIRE(!(flags & LSH_LIGHT_SUN_FRACTION) || sun_fraction, "sun_fraction output missing");
+  LSH_REQUIRE(!(flags & LSH_LIGHT_SUN_MEAN) || sun_mean, "sun_mean output missing");
+  LSH_REQUIRE(!(flags & LSH_LIGHT_EARTH_MARGIN) || earth_margin, "earth_margin output missing");
+  LSH_REQUIRE(!(flags & LSH_LIGHT_EARTH_VISIBLE) || earth_visible, "earth_visible output missing");
+  LSH_REQUIRE(!(flags & LSH_LIGHT_PSR) || psr, "psr output missing");
+  LSH_REQUIRE(!(flags & LSH_LIGHT_PSR_MARGIN) || psr_margin, "psr_margin output missing");
+  if (int r = select_device(s)) return r;
+  std::lock_guard<std::mutex> lock(s->mutex);
+  if (int r = lsh::upload_tiles(s, tiles, tile_count, &a.tiles)) return r;
+  LSH_CHECK_HIP(lsh::launch_lighting(a, s->stream), "lighting kernel launch");
+  return LSH_OK;
+}
+
+int lsh_psr_reduce(const float* elevation, int32_t width, int32_t height,
+                   const double* geo_transform, const double* projection, const double* vectors,
+                   int32_t count, int32_t azimuth_count, int64_t* indices, int32_t* kept) {
+  LSH_REQUIRE(elevation && geo_transform && projection && indices && kept, "invalid arguments");
+  LSH_REQUIRE(width > 0 && height > 0 && azimuth_count > 0, "invalid dimensions");
+  LSH_REQUIRE(count > 0 && vectors, "vectors must be non-empty");
+  for (int64_t i = 0; i < 3 * static_cast<int64_t>(count); ++i)
+    LSH_REQUIRE(std::isfinite(vectors[i]), "vectors must be finite");
+  const double kPi = 3.14159265358979323846;
+  const int points[5][2] = {{0, 0}, {0, width - 1}, {height - 1, 0}, {height - 1, width - 1},
+                            {height / 2, width / 2}};
+  std::vector<int64_t> selected(5 * static_cast<size_t>(azimuth_count), -1);
+  std::vector<double> best(5 * static_cast<size_t>(azimuth_count), -HUGE_VAL);
+  for (int k = 0; k < 5; ++k) {
+    int row = points[k][0], column = points[k][1];
+    const double* t = geo_transform;
+    const double* p = projection;
+    double x = t[0] + t[1] * column + t[2] * row, y = t[3] + t[4] * column + t[5] * row;
+    double xp = x - p[4], yp = y - p[5], rho = std::hypot(xp, yp), lat, lon;
+    if (rho < 1e-9) {
+      lat = p[1];
+      lon = p[2];
+    } else {
+      double c = 2.0 * std::atan2(rho, 2.0 * p[3] * p[0]);
+      lat = std::asin(std::cos(c) * std::sin(p[1]) + yp * std::sin(c) * std::cos(p[1]) / rho);
+      lon = p[2] + std::atan2(xp * std::sin(c),
+                              rho * std::cos(p[1]) * std::cos(c) - yp * std::sin(p[1]) * std::sin(c));
+    }
+    double cl = std::cos(lat), sl = std::sin(lat), co = std::cos(lon), so = std::sin(lon);
+    double basis[3][3] = {{-so, co, 0.0}, {-sl * co, -sl * so, cl}, {cl * co, cl * so, sl}};
+    double radius = p[0] + static_cast<double>(elevation[static_cast<int64_t>(row) * width + column]);
+    double obs[3] = {radius * basis[2][0], radius * basis[2][1], radius * basis[2][2]};
+    double tr[3];
+    for (int j = 0; j < 3; ++j)
+      tr[j] = -(obs[0] * basis[j][0] + obs[1] * basis[j][1] + obs[2] * basis[j][2]);
+    for (int i = 0; i < count; ++i) {
+      const double* v = vectors + 3 * static_cast<int64_t>(i);
+      double e[3];
+      for (int j = 0; j < 3; ++j) e[j] = v[0] * basis[j][0] + v[1] * basis[j][1] + v[2] * basis[j][2] + tr[j];
+      double az = std::fmod(std::atan2(e[0], e[1]) * (180.0 / kPi), 360.0);
+      if (az < 0.0) az += 360.0;
+      double el = std::atan2(e[2], std::hypot(e[0], e[1])) * (180.0 / kPi);
+      int64_t bin = static_cast<int64_t>(az * (azimuth_count / 360.0)) % azimuth_count;
+      size_t idx = static_cast<size_t>(k) * azimuth_count + bin;
+      if (el > best[idx]) {
+        best[idx] = el;
+        selected[idx] = i;
+      }
+    }
+  }
+  std::vector<char> seen(count, 0);
+  int32_t n = 0;
+  for (int64_t v : selected)
+    if (v >= 0 && !seen[v]) {
+      seen[v] = 1;
+      indices[n++] = v;
+    }
+  *kept = n;
+  return LSH_OK;
 }
 
 }  // extern "C"

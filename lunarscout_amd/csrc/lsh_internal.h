@@ -64,8 +64,16 @@ struct Session {
   // session-owned scratch for the one-call tile path
   float* segment_scratch = nullptr;
   size_t segment_scratch_bytes = 0;
+  lsh_tile* tile_scratch = nullptr;  // device copy of a call's tile list
+  int32_t tile_scratch_count = 0;
   int cu_count = 0;
 };
+
+// Sun-disk slice table (lightmap.py:15-29) for a half angle in degrees.
+void sun_disk(double half_angle_deg, float* half_circle, float* max_photons);
+
+// Copies a host tile list into session scratch (grow-only, stream ordered).
+int upload_tiles(Session* s, const lsh_tile* tiles, int32_t count, const lsh_tile** device_tiles);
 
 // Kernel launchers (defined in the .hip files).
 hipError_t launch_pyramid_level(const float* src, int sw, int sh, float* dst, int dw, int dh,
@@ -90,6 +98,37 @@ struct SegmentArgs {
 };
 
 hipError_t launch_segments(const SegmentArgs& args, hipStream_t stream);
+
+struct LightArgs {
+  const float* elevation;  // device level 0 of the DEM the tiles belong to
+  int32_t dem_width, dem_height;
+  double geo_transform[6];
+  double projection[6];
+  const lsh_tile* tiles;   // device [tile_count]
+  int32_t tile_count;
+  const float* horizons;   // device [tile][pixel][N] or [tile][N][pixel], degrees
+  int32_t N;
+  int32_t azimuth_major;
+  const double* sun;       // device [sun_count][3]
+  int32_t sun_count;
+  const double* earth;     // device [earth_count][3]
+  int32_t earth_count;
+  const double* psr_vectors;  // device [psr_count][3]
+  int32_t psr_count;
+  float half_circle[16];   // _HALF_CIRCLE (lightmap.py:19-25)
+  float max_photons;       // _MAX_PHOTONS (lightmap.py:26-29)
+  float sun_half_angle;    // SUN_HALF_ANGLE_DEG (lightmap.py:15)
+  double psr_threshold;    // -diameter / 2 (psr.py:115)
+  uint8_t* sun_bytes;      // [tile][sun_count][16384]
+  float* sun_fraction;     // [tile][sun_count][16384]
+  float* sun_mean;         // [tile][16384]
+  float* earth_margin;     // [tile][earth_count][16384]
+  float* earth_visible;    // [tile][16384]
+  uint8_t* psr;            // [tile][16384]
+  float* psr_margin;       // [tile][16384]
+};
+
+hipError_t launch_lighting(const LightArgs& args, hipStream_t stream);
 hipError_t launch_march(const MarchArgs& args, hipStream_t stream);
 
 // Fills a MarchDem from a registered DEM (float32 parameters the kernel reads).

@@ -20,8 +20,9 @@
 //   * the march is one flat loop: each iteration is one level step of the
 //     hierarchy for every lane, so lanes at different levels stay converged.
 //
-// Arithmetic: the schedule state (ray distance, position, steps) in float64,
-// heights and slopes in float32, as laid out in lsh_march_core.h.
+// Arithmetic: float64 for the ray geometry and, by default, for heights and
+// slopes too (DEM storage stays float32); LSH_HORIZON_F32_VALUES selects
+// float32 heights and slopes (see lsh_march_core.h for the trade-off).
 
 #include "lsh_internal.h"
 #include "lsh_march_core.h"
@@ -32,7 +33,11 @@ namespace {
 constexpr int kBlockPixels = 8;       // 8x8 pixel block per wavefront
 constexpr int kAzimuthsPerGroup = 4;  // wavefronts per workgroup
 
-template <bool kCap, bool kCount>
+// The traversal counters are always accumulated: besides feeding the
+// roofline, their per-branch updates keep the compiler from splitting the flat
+// march loop into nested divergent loops (that restructured form ran 9x slower
+// on gfx950 with identical results). They are stored only when requested.
+template <class V, bool kCap>
 __global__ void __launch_bounds__(256) march_kernel(MarchArgs args) {
   __shared__ int32_t s_off[kMaxDems][kMaxLevels];
   __shared__ int32_t s_w[kMaxDems][kMaxLevels];
@@ -73,11 +78,10 @@ __global__ void __launch_bounds__(256) march_kernel(MarchArgs args) {
   const PixelSelection sel = lsh_select(column, row, args.tile_column, args.tile_row,
                                         args.tile_width, args.subpatch, primary.width,
                                         primary.height);
-  const float oz = lsh_observer_terrain(primary.level0, primary.width, primary.height,
-                                        static_cast<float>(args.tile_column + column),
-                                        static_cast<float>(args.tile_row + row)) +
-                   args.observer_elevation;
-  float current = -1e30f;
+  const V oz = lsh_observer_terrain<V>(primary.level0, primary.width, primary.height,
+                                       args.tile_column + column, args.tile_row + row) +
+               static_cast<V>(args.observer_elevation);
+  V current = V(-1e30);
   int32_t best = -1;
   MarchCounters cnt{0, 0, 0};
   const int64_t seg_stride = static_cast<int64_t>(args.D) * kSegmentFields;
@@ -91,26 +95,31 @@ __global__ void __launch_bounds__(256) march_kernel(MarchArgs args) {
                                           primary.column_resolution / d.column_resolution);
     MarchView view{d.level0, d.mips, s_off[pass], s_w[pass], s_h[pass], d.width, d.height,
                    d.level_count, d.map_resolution, d.radius};
-    lsh_march_pass<double, float, kCap, kCount>(g, view, oz, pass, current, best, cnt);
+    lsh_march_pass<double, V, kCap, true>(g, view, oz, pass, current, best, cnt);
   }
-  float result = current > -1e29f ? current : -INFINITY;
+  float result = current > V(-1e29) ? static_cast<float>(current) : -INFINITY;
   if (args.flags & LSH_HORIZON_DEGREES) {
     // HorizonBuffers.degrees (contract.py:362-371)
     result = isinf(result) ? result : atanf(result) * (180.0f / 3.14159265358979323846f);
   }
   args.output[out_index] = result;
   if (args.winners) args.winners[out_index] = best;
-  if (kCount) {
+  if (args.counters) {
+    // counters[3]: 64 x the wavefront's longest lane (iterations the SIMD issues)
     unsigned long long a = cnt.samples, b = cnt.tests, c = cnt.steps;
+    unsigned m = cnt.steps;
     for (int o = 32; o > 0; o >>= 1) {
       a += __shfl_down(a, o);
       b += __shfl_down(b, o);
       c += __shfl_down(c, o);
+      unsigned other = __shfl_xor(m, o);
+      m = other > m ? other : m;
     }
     if ((threadIdx.x & 63) == 0) {
       atomicAdd(&args.counters[0], a);
       atomicAdd(&args.counters[1], b);
       atomicAdd(&args.counters[2], c);
+      atomicAdd(&args.counters[3], 64ull * m);
     }
   }
 }
@@ -141,21 +150,24 @@ MarchDem march_dem(const Dem& dem) {
   return m;
 }
 
+template <class V>
+void launch_march_values(MarchArgs args, dim3 grid, dim3 block, hipStream_t stream) {
+  if (!(args.flags & LSH_HORIZON_COUNTERS)) args.counters = nullptr;
+  if (args.flags & LSH_HORIZON_CAP)
+    hipLaunchKernelGGL((march_kernel<V, true>), grid, block, 0, stream, args);
+  else
+    hipLaunchKernelGGL((march_kernel<V, false>), grid, block, 0, stream, args);
+}
+
 hipError_t launch_march(const MarchArgs& args, hipStream_t stream) {
   int nbx = (args.tile_width + kBlockPixels - 1) / kBlockPixels;
   int nby = (args.tile_height + kBlockPixels - 1) / kBlockPixels;
   int groups = (args.N + kAzimuthsPerGroup - 1) / kAzimuthsPerGroup;
   dim3 grid(nbx * nby * groups), block(64 * kAzimuthsPerGroup);
-  bool cap = args.flags & LSH_HORIZON_CAP;
-  bool count = (args.flags & LSH_HORIZON_COUNTERS) && args.counters;
-  if (cap && count)
-    hipLaunchKernelGGL((march_kernel<true, true>), grid, block, 0, stream, args);
-  else if (cap)
-    hipLaunchKernelGGL((march_kernel<true, false>), grid, block, 0, stream, args);
-  else if (count)
-    hipLaunchKernelGGL((march_kernel<false, true>), grid, block, 0, stream, args);
+  if (args.flags & LSH_HORIZON_F32_VALUES)
+    launch_march_values<float>(args, grid, block, stream);
   else
-    hipLaunchKernelGGL((march_kernel<false, false>), grid, block, 0, stream, args);
+    launch_march_values<double>(args, grid, block, stream);
   return hipGetLastError();
 }
 

@@ -128,6 +128,24 @@ LSH_HD V lsh_sphere_slope(V hd, V oz, V distance, V radius, V two_observer_radiu
   return lz / lx;
 }
 
+// possible <= current for the cull test (cuda_backend.py:767-802) without
+// forming the quotient: the bound's slope is lz / lx with lx = sqrt(sq - lz^2)
+// (or 1e-6), and the comparison is decided on lz, lx^2 and current^2. Exact
+// arithmetic gives the same answer; rounding can differ only at ties.
+template <class V>
+LSH_HD bool lsh_bound_cannot_rise(V hd, V distance, bool near, V oz, V radius,
+                                  V two_observer_radius, V current) {
+  if (near) return hd <= current * distance;  // distance >= 1 m
+  V sq = distance * distance;
+  V rs = (V(2) * radius + V(2) * oz) + hd;
+  V lz = (hd * rs - sq) / two_observer_radius;
+  V lxx = sq - lz * lz;
+  if (!(lxx > V(0))) return lz * V(1e6) <= current;
+  V rhs = current * current * lxx;
+  if (lz <= V(0)) return current >= V(0) || lz * lz >= rhs;
+  return current > V(0) && lz * lz <= rhs;
+}
+
 // Ray distance s in km. Single precision keeps an unevaluated pair (hi + lo)
 // updated by TwoSum; double precision keeps one value.
 template <class S>
@@ -250,12 +268,11 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     }
     const V true_near = static_cast<V>(lsh_max(true_m - static_cast<S>(scale) * res, S(1)));
     const V hmax = static_cast<V>(maxh) - oz;
-    const V possible = near ? hmax / true_near
-                            : lsh_sphere_slope(hmax, oz, true_near, radius, two_observer_radius);
-    if (possible <= current) {  // cull
+    if (lsh_bound_cannot_rise(hmax, true_near, near, oz, radius, two_observer_radius, current)) {
+      // cull
       if (kCount) ++cnt.tests;
       trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx), float(py + g.by),
-                maxh, 0.f, float(possible), float(skip), 1);
+                maxh, 0.f, 0.f, float(skip), 1);
       s.add(skip);
       need_position = true;
       continue;
@@ -299,7 +316,7 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     }
     if (kCount) ++cnt.tests;
     trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx), float(py + g.by),
-              maxh, 0.f, float(possible), 0.f, 0);
+              maxh, 0.f, 0.f, 0.f, 0);
     --level;
   }
 }
@@ -311,18 +328,23 @@ LSH_HD int lsh_clamp_center(int requested, int dem_size, int sub) {
   return requested < half ? half : (requested > maximum ? maximum : requested);
 }
 
-// bilinear (cuda_backend.py:107-132) of a DEM at an integer observer pixel.
-LSH_HD float lsh_observer_terrain(const float* e, int w, int h, float column, float row) {
-  column = __builtin_fminf(__builtin_fmaxf(column, 0.0f), static_cast<float>(w) - 1.0001f);
-  row = __builtin_fminf(__builtin_fmaxf(row, 0.0f), static_cast<float>(h) - 1.0001f);
-  int x0 = static_cast<int>(__builtin_floorf(column)), y0 = static_cast<int>(__builtin_floorf(row));
+// bilinear (cuda_backend.py:107-132) of a DEM at an integer observer pixel, in
+// precision V (the clamp at size - 1.0001 interpolates the last row/column).
+template <class V>
+LSH_HD V lsh_observer_terrain(const float* e, int w, int h, int column_i, int row_i) {
+  V column = static_cast<V>(column_i), row = static_cast<V>(row_i);
+  const V cmax = static_cast<V>(w) - V(1.0001);
+  const V rmax = static_cast<V>(h) - V(1.0001);
+  column = lsh_min(lsh_max(column, V(0)), cmax);
+  row = lsh_min(lsh_max(row, V(0)), rmax);
+  int x0 = static_cast<int>(lsh_floor(column)), y0 = static_cast<int>(lsh_floor(row));
   int x1 = x0 + 1 < w ? x0 + 1 : w - 1, y1 = y0 + 1 < h ? y0 + 1 : h - 1;
   float q00 = e[static_cast<int64_t>(y0) * w + x0], q10 = e[static_cast<int64_t>(y0) * w + x1];
   float q01 = e[static_cast<int64_t>(y1) * w + x0], q11 = e[static_cast<int64_t>(y1) * w + x1];
-  if (!(lsh_valid(q00) && lsh_valid(q10) && lsh_valid(q01) && lsh_valid(q11))) return -32000.0f;
-  float tx = column - static_cast<float>(x0), ty = row - static_cast<float>(y0);
-  float top = q00 + tx * (q10 - q00);
-  float bottom = q01 + tx * (q11 - q01);
+  if (!(lsh_valid(q00) && lsh_valid(q10) && lsh_valid(q01) && lsh_valid(q11))) return V(-32000);
+  V tx = column - static_cast<V>(x0), ty = row - static_cast<V>(y0);
+  V top = V(q00) + tx * (V(q10) - V(q00));
+  V bottom = V(q01) + tx * (V(q11) - V(q01));
   return top + ty * (bottom - top);
 }
 

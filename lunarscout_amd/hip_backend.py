@@ -206,7 +206,8 @@ class HipSession(_lib.Session):
                      cap: bool = False, counters: bool = False):
         """Segments + march of one 128x128 tile on the GPU; host float32 [16384, N].
 
-        With ``counters`` returns (values, int64[3] level-0 samples / bound tests / steps).
+        With ``counters`` returns (values, int64[4] level-0 samples / bound tests / steps /
+        64 x longest-lane steps per wavefront).
         """
         self._segments = self.segments_device(
             dems, tile_column=tile_column, tile_row=tile_row, azimuth_count=azimuth_count,
@@ -215,7 +216,7 @@ class HipSession(_lib.Session):
         flags = (_lib.HORIZON_DEGREES if degrees else 0) | (_lib.HORIZON_CAP if cap else 0)
         counter_buf = None
         if counters:
-            counter_buf = DeviceBuffer.empty(self, (3,), np.int64)
+            counter_buf = DeviceBuffer.empty(self, (4,), np.int64)
             counter_buf.zero()
             flags |= _lib.HORIZON_COUNTERS
         self._output = self.march_device(dems, self._segments, tile_column=tile_column,

@@ -49,10 +49,8 @@ float ray(const kc_pyramid* pyrs, int D, const float* segments, int C, int az, i
   const kc_dem& primary = pyrs[0].dem;
   lsh::PixelSelection sel = lsh::lsh_select(column, row, tile_column, tile_row, tile_width,
                                             subpatch, primary.width, primary.height);
-  float ozf = lsh::lsh_observer_terrain(primary.elevation, primary.width, primary.height,
-                                        float(tile_column + column), float(tile_row + row)) +
-              obs_h;
-  V oz = V(ozf);
+  V oz = lsh::lsh_observer_terrain<V>(primary.elevation, primary.width, primary.height,
+                                      tile_column + column, tile_row + row) + V(obs_h);
   V current = V(-1e30);
   int32_t best = -1;
   int64_t stride = int64_t(D) * 18;
