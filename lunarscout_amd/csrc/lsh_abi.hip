@@ -304,6 +304,14 @@ int lsh_dem_create(lsh_session* session, const float* elevation, int32_t width, 
   d.height = height;
   std::memcpy(d.geo_transform, geo_transform, sizeof(d.geo_transform));
   std::memcpy(d.projection, projection, sizeof(d.projection));
+  d.all_valid = 1;
+  for (int64_t i = 0, n = static_cast<int64_t>(width) * height; i < n; ++i) {
+    float v = elevation[i];
+    if (!(std::isfinite(v) && v > -20000.0f)) {
+      d.all_valid = 0;
+      break;
+    }
+  }
   if (int r = select_device(s)) return r;
   std::lock_guard<std::mutex> lock(s->mutex);
   size_t level0_bytes = sizeof(float) * static_cast<size_t>(width) * height;

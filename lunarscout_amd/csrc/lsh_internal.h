@@ -29,6 +29,7 @@ struct Dem {
   int64_t mip_count = 0;
   int32_t level_count = 0;
   int32_t levels[kMaxLevels][4] = {{0}};  // (level, offset, width, height)
+  int32_t all_valid = 0;  // every elevation finite and above -20000 m
 };
 
 // Per-pass description of a DEM as the march kernel reads it.
@@ -39,6 +40,7 @@ struct MarchDem {
   float map_resolution;     // (column + row resolution) / 2 (cuda_backend.py:1124-1127)
   float column_resolution;  // hypot(t1, t4) (cuda_backend.py:1032-1035)
   float radius;             // projection radius (cuda_backend.py:1132)
+  int32_t all_valid;
   int32_t level_offset[kMaxLevels];
   int32_t level_width[kMaxLevels];
   int32_t level_height[kMaxLevels];

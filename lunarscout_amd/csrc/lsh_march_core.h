@@ -51,6 +51,7 @@ struct MarchView {
   int32_t width, height, level_count;
   float map_resolution;  // (column + row resolution) / 2
   float radius;
+  int32_t all_valid;     // 1 when every level-0 elevation is valid (no nodata tests needed)
 };
 
 // One pass's interpolated segment with its start relative to (bx, by), in the
@@ -134,11 +135,11 @@ LSH_HD V lsh_sphere_slope(V hd, V oz, V distance, V radius, V two_observer_radiu
 // arithmetic gives the same answer; rounding can differ only at ties.
 template <class V>
 LSH_HD bool lsh_bound_cannot_rise(V hd, V distance, bool near, V oz, V radius,
-                                  V two_observer_radius, V current) {
+                                  V inv_two_observer_radius, V current) {
   if (near) return hd <= current * distance;  // distance >= 1 m
   V sq = distance * distance;
   V rs = (V(2) * radius + V(2) * oz) + hd;
-  V lz = (hd * rs - sq) / two_observer_radius;
+  V lz = (hd * rs - sq) * inv_two_observer_radius;
   V lxx = sq - lz * lz;
   if (!(lxx > V(0))) return lz * V(1e6) <= current;
   V rhs = current * current * lxx;
@@ -170,11 +171,36 @@ struct RayDistance {
   LSH_HD S value() const { return hi + lo; }
 };
 
+// Distance along the local tangent to the exit of the current cell, in km
+// (cuda_backend.py:733-759): min over the axes of (far edge - p) / d, where an
+// axis with |d| <= 1e-8 uses the inverse 1e30. The axes are compared by
+// cross-multiplication, then the winner takes one division.
+template <class S>
+LSH_HD S lsh_cell_exit(S ex0, S ey0, S scale, S dx, S dy) {
+  const S adx = lsh_abs(dx), ady = lsh_abs(dy);
+  const bool tiny_x = !(adx > S(1e-8)), tiny_y = !(ady > S(1e-8));
+  // (edge - p) for the edge the tangent points at; ex0 = min edge - p <= 0
+  const S ax = tiny_x || dx > S(0) ? ex0 + scale : -ex0;
+  const S ay = tiny_y || dy > S(0) ? ey0 + scale : -ey0;
+  // an axis with a tiny tangent never wins unless both are tiny
+  const bool use_x = tiny_y ? (tiny_x ? ax <= ay : true) : (tiny_x ? false : ax * ady <= ay * adx);
+  const S num = use_x ? ax : ay;
+  const S den = tiny_x && tiny_y ? S(1e-30) : (use_x ? adx : ady);
+  return num / den;
+}
+
 // hierarchy_maximum (cuda_backend.py:656-874) for one DEM pass. S is the
 // precision of the schedule state (ray distance, position, tangent, distances,
 // steps); V the precision of heights and slopes. `current` is the running
 // maximum slope (-1e30 when none yet); `best` receives
 // (pass << 24) | level-0 sample index when a sample raises it.
+//
+// One outer iteration is one position of the schedule: evaluate the path, pick
+// the start level, test the 2x2 bound there (descending only while the bound
+// neither culls nor is at level 0, which is rare), then advance s by the cell
+// skip or by the level-0 step. Both the skip and the step are formed for every
+// lane and selected, so a wavefront executes one straight path per position.
+// Constant divisions of the reference (x / 1000) are multiplications by 0.001.
 template <class S, class V, bool kCap, bool kCount, class Tracer = NoTrace>
 LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, int pass,
                            V& current, int32_t& best, MarchCounters& cnt,
@@ -182,6 +208,8 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   const S res = static_cast<S>(d.map_resolution);
   const V radius = static_cast<V>(d.radius);
   const V two_observer_radius = V(2) * (radius + oz);
+  const V inv_two_observer_radius = V(1) / two_observer_radius;
+  const V two_radius_oz = V(2) * radius + V(2) * oz;
   const S min_step = (S(0.5) * res) * S(0.001);
   const S far_step = (S(0.8) * res) * S(0.001);
   // bounds of the reference's absolute test px in [0, W-1), as offsets
@@ -191,133 +219,145 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   // the bilinear clamp at width - 1.0001 (cuda_backend.py:113-114), as a fraction
   const S clamp_tx = (static_cast<S>(d.width) - S(1.0001)) - static_cast<S>(d.width - 2);
   const S clamp_ty = (static_cast<S>(d.height) - S(1.0001)) - static_cast<S>(d.height - 2);
+  const bool all_valid = d.all_valid != 0;
+  const int top_level = d.level_count - 1;
   RayDistance<S> s;
   s.start(lsh_max(g.s_start, S(0.001)));
-  bool need_position = true;
-  int level = 0, sample_index = 0;
-  S px = 0, py = 0, true_m = 0, dx = 0, dy = 0, inv_x = 0, inv_y = 0;
-  int ix = 0, iy = 0;
-  bool near = true;
-  while (true) {
-    if (need_position) {
-      if (!s.at_most(g.s_end)) break;
-      S delta = s.minus(g.s_start);
-      px = lsh_quartic(g.sx, g.a1, g.a2, g.a3, g.a4, delta);
-      py = lsh_quartic(g.sy, g.b1, g.b2, g.b3, g.b4, delta);
-      if (px != px || py != py || px < lo_x || py < lo_y || px >= hi_x || py >= hi_y) break;
-      near = s.below(S(0.5));
-      if (near) {
-        true_m = s.hi * S(1000) + s.lo * S(1000);
-      } else {
-        S plx = (px - g.sx) * res, ply = (py - g.sy) * res;
-        S planar = lsh_sqrt(plx * plx + ply * ply);
-        S p2 = planar * planar, p3 = p2 * planar;
-        S chord = g.c1 * planar;
-        chord = chord + g.c2 * p2;
-        chord = chord + g.c3 * p3;
-        true_m = g.s_chord * S(1000) + chord;
+  int sample_index = 0;
+  while (s.at_most(g.s_end)) {
+    // -- position (cuda_backend.py:672-705)
+    const S delta = s.minus(g.s_start);
+    const S px = lsh_quartic(g.sx, g.a1, g.a2, g.a3, g.a4, delta);
+    const S py = lsh_quartic(g.sy, g.b1, g.b2, g.b3, g.b4, delta);
+    if (px != px || py != py || px < lo_x || py < lo_y || px >= hi_x || py >= hi_y) break;
+    const bool near = s.below(S(0.5));
+    const S plx = (px - g.sx) * res, ply = (py - g.sy) * res;
+    const S planar = lsh_sqrt(plx * plx + ply * ply);
+    const S p2 = planar * planar, p3 = p2 * planar;
+    S chord = g.c1 * planar;
+    chord = chord + g.c2 * p2;
+    chord = chord + g.c3 * p3;
+    const S true_m = near ? s.hi * S(1000) + s.lo * S(1000) : g.s_chord * S(1000) + chord;
+    // start level: the highest level whose cell side fits the beam footprint
+    const S footprint = true_m * static_cast<S>(kBeamWidth);
+    int level = 0;
+    for (int l = 1; l <= top_level; ++l)
+      level += static_cast<S>(1 << (l * 2)) * res <= footprint ? 1 : 0;
+    const S dx = lsh_tangent(g.a1, g.a2, g.a3, g.a4, delta);
+    const S dy = lsh_tangent(g.b1, g.b2, g.b3, g.b4, delta);
+    const S fx = lsh_floor(px), fy = lsh_floor(py);
+    const int ix = g.bx + static_cast<int>(fx), iy = g.by + static_cast<int>(fy);
+    // -- hierarchy tests at this position (cuda_backend.py:706-868)
+    bool ray_out = false, sampled = false;
+    int cx = 0, cy = 0, scale = 1;
+    float v00 = 0.f, v10 = 0.f, v01 = 0.f, v11 = 0.f, maxh = 0.f;
+    bool taps_valid = true;
+    while (true) {
+      const int shift = level * 2;
+      scale = 1 << shift;
+      cx = ix >> shift;
+      cy = iy >> shift;
+      const int w = d.level_width[level], h = d.level_height[level];
+      if (!(cx < w && cy < h)) {  // leaves the level: the ray ends
+        ray_out = true;
+        break;
       }
-      S footprint = true_m * static_cast<S>(kBeamWidth);
-      level = d.level_count - 1;
-      while (level > 0 && static_cast<S>(1 << (level * 2)) * res > footprint) --level;
-      dx = lsh_tangent(g.a1, g.a2, g.a3, g.a4, delta);
-      dy = lsh_tangent(g.b1, g.b2, g.b3, g.b4, delta);
-      inv_x = lsh_abs(dx) > S(1e-8) ? S(1) / dx : S(1e30);
-      inv_y = lsh_abs(dy) > S(1e-8) ? S(1) / dy : S(1e30);
-      ix = g.bx + static_cast<int>(lsh_floor(px));
-      iy = g.by + static_cast<int>(lsh_floor(py));
-      need_position = false;
-    }
-    const int shift = level * 2;
-    const int scale = 1 << shift;
-    const int cx = ix >> shift, cy = iy >> shift;
-    const int w = d.level_width[level], h = d.level_height[level];
-    if (!(cx < w && cy < h)) break;  // leaves the level: the ray ends
-    if (kCount) ++cnt.steps;
-    // hierarchy_bilinear_bound (cuda_backend.py:47-60): 2x2 cells clipped to the level
-    const float* base = level == 0 ? d.level0 : d.mips + d.level_offset[level];
-    const int cx1 = cx + 1 < w ? cx + 1 : w - 1;
-    const int cy1 = cy + 1 < h ? cy + 1 : h - 1;
-    const float* r0 = base + static_cast<int64_t>(cy) * w;
-    const float* r1 = base + static_cast<int64_t>(cy1) * w;
-    const float v00 = r0[cx], v10 = r0[cx1], v01 = r1[cx], v11 = r1[cx1];
-    const bool has_x = cx + 1 < w, has_y = cy + 1 < h;
-    const bool ok00 = lsh_valid(v00), ok10 = has_x && lsh_valid(v10);
-    const bool ok01 = has_y && lsh_valid(v01), ok11 = has_x && has_y && lsh_valid(v11);
-    float maxh = -32000.0f;
-    if (ok00 && v00 > maxh) maxh = v00;
-    if (ok10 && v10 > maxh) maxh = v10;
-    if (ok01 && v01 > maxh) maxh = v01;
-    if (ok11 && v11 > maxh) maxh = v11;
-    // cell exit along the local tangent, in km (cuda_backend.py:733-759)
-    const S ex0 = static_cast<S>(cx * scale - g.bx) - px;
-    const S ey0 = static_cast<S>(cy * scale - g.by) - py;
-    const S t1 = ex0 * inv_x, t2 = (ex0 + static_cast<S>(scale)) * inv_x;
-    const S t3 = ey0 * inv_y, t4 = (ey0 + static_cast<S>(scale)) * inv_y;
-    const S exit_distance = lsh_min(lsh_max(t1, t2), lsh_max(t3, t4));
-    const S fallback = ((static_cast<S>(scale) * res) * S(0.5)) * S(0.001);
-    const S to_exit = exit_distance > S(0) ? exit_distance : fallback;
-    const S skip = to_exit > S(0) ? to_exit + S(0.0001) : fallback;
-    if (maxh < -20000.0f) {  // nodata block
+      if (kCount) ++cnt.steps;
+      // hierarchy_bilinear_bound (cuda_backend.py:47-60): the 2x2 cells clipped to
+      // the level (a clipped neighbour repeats the cell, leaving the max unchanged)
+      const float* base = level == 0 ? d.level0 : d.mips + d.level_offset[level];
+      const int cx1 = cx + 1 < w ? cx + 1 : w - 1;
+      const int cy1 = cy + 1 < h ? cy + 1 : h - 1;
+      const float* r0 = base + cy * w;
+      const float* r1 = base + cy1 * w;
+      v00 = r0[cx];
+      v10 = r0[cx1];
+      v01 = r1[cx];
+      v11 = r1[cx1];
+      if (all_valid) {
+        maxh = __builtin_fmaxf(__builtin_fmaxf(v00, v10), __builtin_fmaxf(v01, v11));
+      } else {
+        const bool has_x = cx + 1 < w, has_y = cy + 1 < h;
+        const bool ok00 = lsh_valid(v00), ok10 = has_x && lsh_valid(v10);
+        const bool ok01 = has_y && lsh_valid(v01), ok11 = has_x && has_y && lsh_valid(v11);
+        taps_valid = ok00 && ok10 && ok01 && ok11;
+        maxh = -32000.0f;
+        if (ok00 && v00 > maxh) maxh = v00;
+        if (ok10 && v10 > maxh) maxh = v10;
+        if (ok01 && v01 > maxh) maxh = v01;
+        if (ok11 && v11 > maxh) maxh = v11;
+      }
+      if (maxh < -20000.0f) {  // nodata block
+        if (kCount) ++cnt.tests;
+        break;
+      }
+      const V true_near = static_cast<V>(lsh_max(true_m - static_cast<S>(scale) * res, S(1)));
+      if (lsh_bound_cannot_rise(static_cast<V>(maxh) - oz, true_near, near, oz, radius,
+                                inv_two_observer_radius, current)) {  // cull
+        if (kCount) ++cnt.tests;
+        break;
+      }
+      if (level == 0) {
+        sampled = true;
+        break;
+      }
       if (kCount) ++cnt.tests;
-      trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx), float(py + g.by),
-                maxh, 0.f, 0.f, float(skip), 2);
-      s.add(skip);
-      need_position = true;
-      continue;
+      trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx),
+                float(py + g.by), maxh, 0.f, 0.f, 0.f, 0);
+      --level;
     }
-    const V true_near = static_cast<V>(lsh_max(true_m - static_cast<S>(scale) * res, S(1)));
-    const V hmax = static_cast<V>(maxh) - oz;
-    if (lsh_bound_cannot_rise(hmax, true_near, near, oz, radius, two_observer_radius, current)) {
-      // cull
-      if (kCount) ++cnt.tests;
-      trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx), float(py + g.by),
-                maxh, 0.f, 0.f, float(skip), 1);
-      s.add(skip);
-      need_position = true;
-      continue;
-    }
-    if (level == 0) {
+    if (ray_out) break;
+    // -- cell skip (cull / nodata) and level-0 step, selected per lane
+    const S fscale = static_cast<S>(scale);
+    const S fallback = ((fscale * res) * S(0.5)) * S(0.001);
+    const S exit_distance = lsh_cell_exit(static_cast<S>(cx * scale - g.bx) - px,
+                                          static_cast<S>(cy * scale - g.by) - py, fscale, dx, dy);
+    const S skip = exit_distance > S(0) ? exit_distance + S(0.0001) : fallback;
+    S advance = skip;
+    if (sampled) {
       // bilinear (cuda_backend.py:107-132) of h - oz on the same 2x2 taps
-      V slope = V(-1e30);
-      if (ok00 && ok10 && ok01 && ok11) {
-        S sx_frac = px - lsh_floor(px), sy_frac = py - lsh_floor(py);
-        if (ix == d.width - 2 && sx_frac > clamp_tx) sx_frac = clamp_tx;
-        if (iy == d.height - 2 && sy_frac > clamp_ty) sy_frac = clamp_ty;
-        const V tx = static_cast<V>(sx_frac), ty = static_cast<V>(sy_frac);
-        const V d00 = V(v00) - oz, d10 = V(v10) - oz, d01 = V(v01) - oz, d11 = V(v11) - oz;
-        const V top = d00 + tx * (d10 - d00);
-        const V bottom = d01 + tx * (d11 - d01);
-        const V hd = top + ty * (bottom - top);
-        const V tm = static_cast<V>(true_m);
-        slope = near ? hd / tm : lsh_sphere_slope(hd, oz, tm, radius, two_observer_radius);
-        if (slope > current) {
-          current = slope;
-          best = (pass << 24) | sample_index;
-        }
+      S sx_frac = px - fx, sy_frac = py - fy;
+      if (ix == d.width - 2 && sx_frac > clamp_tx) sx_frac = clamp_tx;
+      if (iy == d.height - 2 && sy_frac > clamp_ty) sy_frac = clamp_ty;
+      const V tx = static_cast<V>(sx_frac), ty = static_cast<V>(sy_frac);
+      const V d00 = V(v00) - oz, d10 = V(v10) - oz, d01 = V(v01) - oz, d11 = V(v11) - oz;
+      const V top = d00 + tx * (d10 - d00);
+      const V bottom = d01 + tx * (d11 - d01);
+      const V hd = top + ty * (bottom - top);
+      const V tm = static_cast<V>(true_m);
+      // slope: (h - oz) / d near the observer, lz / lx on the sphere (cuda_backend.py:806-830)
+      const V sq = tm * tm;
+      const V lz = (hd * (two_radius_oz + hd) - sq) * inv_two_observer_radius;
+      const V lxx = sq - lz * lz;
+      const V num = near ? hd : lz;
+      const V den = near ? tm : (lxx > V(0) ? lsh_sqrt(lxx) : V(1e-6));
+      V slope = num / den;
+      if (!taps_valid) slope = V(-1e30);
+      if (slope > current) {
+        current = slope;
+        best = (pass << 24) | sample_index;
       }
       if (kCount) ++cnt.samples;
       ++sample_index;
+      // adaptive level-0 advance (cuda_backend.py:833-861)
       const S mag = lsh_sqrt(dx * dx + dy * dy);
       const S pixel_step = mag > S(1e-6) ? S(1) / mag : S(0.0005);
       const S margin = static_cast<S>(current - slope);
-      const S margin_step = margin > S(0) ? ((margin * true_m) * S(1.732)) / S(1000) : S(0);
-      const S angular_step = (true_m * S(0.00151)) / S(1000);
-      S advance = lsh_max(pixel_step, lsh_min(margin_step, angular_step));
-      if (near) advance = advance * S(0.25);
+      const S margin_step = margin > S(0) ? ((margin * true_m) * S(1.732)) * S(0.001) : S(0);
+      const S angular_step = (true_m * S(0.00151)) * S(0.001);
+      S step = lsh_max(pixel_step, lsh_min(margin_step, angular_step));
+      if (near) step = step * S(0.25);
       const S floor_step = (pass == 0 && true_m >= S(100)) ? far_step : min_step;
-      advance = lsh_max(advance, floor_step);
-      if (kCap) advance = lsh_min(advance, to_exit > S(0) ? to_exit + S(0.000001) : fallback);
-      trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx), float(py + g.by),
-                maxh, 0.f, float(slope), float(advance), 4);
-      s.add(advance);
-      need_position = true;
-      continue;
+      step = lsh_max(step, floor_step);
+      if (kCap) step = lsh_min(step, exit_distance > S(0) ? exit_distance + S(0.000001) : fallback);
+      advance = step;
+      trace.row(float(s.value()), float(true_m), 0, cx, cy, float(px + g.bx), float(py + g.by),
+                maxh, float(hd), float(slope), float(advance), 4);
+    } else {
+      trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx),
+                float(py + g.by), maxh, 0.f, 0.f, float(skip), maxh < -20000.0f ? 2 : 1);
     }
-    if (kCount) ++cnt.tests;
-    trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx), float(py + g.by),
-              maxh, 0.f, 0.f, 0.f, 0);
-    --level;
+    s.add(advance);
   }
 }
 

@@ -16,6 +16,7 @@ namespace {
 struct PassTables {
   int32_t off[4][16], lw[4][16], lh[4][16];
   float colres[4], mapres[4];
+  int32_t all_valid[4];
 };
 }  // namespace
 
@@ -62,7 +63,7 @@ float ray(const kc_pyramid* pyrs, int D, const float* segments, int C, int az, i
         base + sel.i11 * stride, sel, column, row, t.colres[0] / t.colres[p]);
     lsh::MarchView v{pyrs[p].dem.elevation, pyrs[p].mips, t.off[p], t.lw[p], t.lh[p],
                      pyrs[p].dem.width, pyrs[p].dem.height, pyrs[p].level_count, t.mapres[p],
-                     float(pyrs[p].dem.projection[0])};
+                     float(pyrs[p].dem.projection[0]), t.all_valid[p]};
     if (cap)
       lsh::lsh_march_pass<S, V, true, true>(g, v, oz, p, current, best, cnt, tr);
     else
@@ -85,6 +86,13 @@ PassTables tables(const kc_pyramid* pyrs, int D) {
     float t1 = float(g[1]), t4 = float(g[4]), t2 = float(g[2]), t5 = float(g[5]);
     t.colres[p] = std::sqrt(t1 * t1 + t4 * t4);
     t.mapres[p] = (t.colres[p] + std::sqrt(t2 * t2 + t5 * t5)) * 0.5f;
+    t.all_valid[p] = 1;
+    const float* e = pyrs[p].dem.elevation;
+    for (int64_t i = 0, n = int64_t(pyrs[p].dem.width) * pyrs[p].dem.height; i < n; ++i)
+      if (!(std::isfinite(e[i]) && e[i] > -20000.0f)) {
+        t.all_valid[p] = 0;
+        break;
+      }
   }
   return t;
 }
