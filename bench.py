@@ -9,10 +9,13 @@ passes, then from those horizons the lightmap (finite-disk sun fraction bytes
 per time), the Earth-visibility fraction and the PSR mask for 125 Sun and 125
 Earth Moon-ME vectors at 6-hour steps over 31 days.
 
-A step is one batch of tiles (``--tiles-per-step``, default 32): FP64 segment
+A step is one batch of tiles (``--tiles-per-step``, default 24): FP64 segment
 generation, the fused two-pass march into HBM (degrees), and the streamed
-lighting pass over the batch. The default 8 timed steps cover the 256 tiles
-once. DEMs and vectors are resident in HBM before the timed region.
+lighting pass over the batch. Tiles are taken in the order i * 97 mod 256, so
+any run of consecutive steps samples rows and columns of the whole site evenly;
+the default 4 timed steps cover 96 of the 256 tiles (``--steps 11 --tiles-per-step
+24`` or ``--steps 8 --tiles-per-step 32`` covers the site). DEMs and vectors are
+resident in HBM before the timed region.
 
 Prints one JSON line (see the fields below).
 """
@@ -44,13 +47,15 @@ def log(*args):
 def parse_args():
     p = argparse.ArgumentParser(description=__doc__)
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=8)
+    p.add_argument("--steps", type=int, default=4)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--tiles-per-step", type=int, default=32)
+    p.add_argument("--tiles-per-step", type=int, default=24)
     p.add_argument("--cpu-sample-pixels", type=int, default=256,
                    help="pixels of one tile the CPU baseline marches (0 disables it)")
     p.add_argument("--counter-tiles", type=int, default=4,
-                   help="tiles re-run with traversal counters for the roofline")
+                   help="tiles re-run with traversal counters for the roofline (0 skips)")
+    p.add_argument("--no-probes", action="store_true",
+                   help="skip the copy / gather peak probes")
     return p.parse_args()
 
 
@@ -83,7 +88,8 @@ def main() -> int:
     earth_d = DeviceBuffer.from_host(session, earth)
     psr_d = DeviceBuffer.from_host(session, psr_vectors)
 
-    tiles = [(x, y) for y in range(0, primary.height, 128) for x in range(0, primary.width, 128)]
+    grid = [(x, y) for y in range(0, primary.height, 128) for x in range(0, primary.width, 128)]
+    tiles = [grid[(i * 97) % len(grid)] for i in range(len(grid))]
     per_step = args.tiles_per_step
     horizons = DeviceBuffer.empty(session, (per_step, PIXELS, AZIMUTHS), np.float32)
     segments = None
@@ -134,24 +140,33 @@ def main() -> int:
     lighting_ms = stats["lighting_ms"] / max(stats["lighting_launches"], 1)
     light_px_times = per_step * PIXELS * (len(sun) + len(earth) + len(psr_vectors))
 
-    # traversal counters on a spread of tiles -> algorithmic bytes per tile
-    counters = DeviceBuffer.empty(session, (4,), np.int64)
-    samples = tests = 0
-    pick = [tiles[i] for i in np.linspace(0, len(tiles) - 1, args.counter_tiles).astype(int)]
-    for c, r in pick:
-        counters.zero()
-        segments = session.segments_device(dems, tile_column=c, tile_row=r, out=segments)
-        session.march_device(dems, segments, tile_column=c, tile_row=r, flags=HORIZON_COUNTERS,
-                             output=_slot_view(horizons, 0), counters=counters)
-        v = counters.to_host()
-        samples += int(v[0])
-        tests += int(v[1])
-    rays = PIXELS * AZIMUTHS
-    S, T = samples / len(pick), tests / len(pick)
-    passes = len(dems)
-    bytes_alg = 16.0 * (S + T) + 256.0 * rays * passes + 4.0 * rays
-    achieved = bytes_alg / (march_ms / 1000.0) / 1e9
+    # traversal counters on the timed tiles -> algorithmic bytes per tile
+    S = T = bytes_alg = achieved = None
+    if args.counter_tiles > 0:
+        counters = DeviceBuffer.empty(session, (4,), np.int64)
+        samples = tests = 0
+        timed = tiles[args.warmup * per_step:(args.warmup + args.steps) * per_step] or tiles
+        pick = [timed[i] for i in np.linspace(0, len(timed) - 1, args.counter_tiles).astype(int)]
+        for c, r in pick:
+            counters.zero()
+            segments = session.segments_device(dems, tile_column=c, tile_row=r, out=segments)
+            session.march_device(dems, segments, tile_column=c, tile_row=r, flags=HORIZON_COUNTERS,
+                                 output=_slot_view(horizons, 0), counters=counters)
+            v = counters.to_host()
+            samples += int(v[0])
+            tests += int(v[1])
+        rays = PIXELS * AZIMUTHS
+        S, T = samples / len(pick), tests / len(pick)
+        passes = len(dems)
+        bytes_alg = 16.0 * (S + T) + 256.0 * rays * passes + 4.0 * rays
+        achieved = bytes_alg / (march_ms / 1000.0) / 1e9
     traffic = _traffic_from_profile()
+    peaks = None
+    if not args.no_probes:
+        from lunarscout_amd.profiling import probe_peaks
+
+        horizons.free()
+        peaks = probe_peaks(session)
 
     result = {
         "metric": METRIC,
@@ -164,7 +179,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (heights, slopes) + f64 (ray geometry)",
+        "dtype": "f64 (ray geometry, heights, slopes; DEMs stored f32; lighting f32 disk slices)",
         "data": "synthetic (seeded fBm + craters DEMs, synthetic Moon-ME vectors)",
         "config": {
             "workload": "full site, plain HIP path: 2048^2 @5 m primary + 4096^2 @60 m regional, "
@@ -187,12 +202,15 @@ def main() -> int:
         "roofline": {
             "kernel": "march_kernel",
             "bound": "hbm",
-            "achieved": round(achieved, 1),
+            "achieved": round(achieved, 1) if achieved is not None else None,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved is not None else None,
             "traffic": traffic,
             "bytes_alg_per_launch": bytes_alg,
+            "measured_peaks": peaks,
+            "frac_of_l2_gather": (round(achieved / peaks["gather_l2_gbs"], 4)
+                                  if peaks and achieved is not None else None),
         },
     }
     if args.cpu_sample_pixels > 0:

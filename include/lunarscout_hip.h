@@ -186,6 +186,19 @@ int lsh_psr_reduce(const float* elevation, int32_t width, int32_t height,
  * half_circle float32[16] and max_photons. */
 int lsh_sun_disk(double sun_half_angle_deg, float* half_circle, float* max_photons);
 
+/* Measurement probes (no counterpart in the reference; they calibrate the
+ * roofline of the march on the running device).
+ * lsh_probe_copy: device-to-device copy of `bytes` (a multiple of 16) by a
+ * streaming float4 kernel -- the known-size stream that scales rocprofv3's
+ * FETCH_SIZE / WRITE_SIZE counters.
+ * lsh_probe_gather: `threads` (a multiple of 256) threads each take `taps`
+ * random 2x2 float32 taps of a width x height device window (width, height
+ * >= 2) and write their maximum to out[threads]; the algorithmic gather rate is
+ * 16 * threads * taps bytes over the kernel time. */
+int lsh_probe_copy(lsh_session* session, void* device_dst, const void* device_src, size_t bytes);
+int lsh_probe_gather(lsh_session* session, const float* window, int32_t width, int32_t height,
+                     int32_t taps, int32_t threads, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -625,4 +625,30 @@ int lsh_psr_reduce(const float* elevation, int32_t width, int32_t height,
   return LSH_OK;
 }
 
+int lsh_probe_copy(lsh_session* session, void* device_dst, const void* device_src, size_t bytes) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(s && device_dst && device_src, "invalid arguments");
+  LSH_REQUIRE(bytes % 16 == 0, "bytes must be a multiple of 16");
+  if (int r = select_device(s)) return r;
+  LSH_CHECK_HIP(lsh::launch_probe_copy(static_cast<const float*>(device_src),
+                                  static_cast<float*>(device_dst), static_cast<int64_t>(bytes),
+                                  s->stream),
+                "copy probe launch");
+  return LSH_OK;
+}
+
+int lsh_probe_gather(lsh_session* session, const float* window, int32_t width, int32_t height,
+                     int32_t taps, int32_t threads, float* out) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(s && window && out, "invalid arguments");
+  LSH_REQUIRE(width >= 2 && height >= 2 && width <= (1 << 24) && height <= (1 << 24),
+              "window must be at least 2x2 and at most 2^24 on a side");
+  LSH_REQUIRE(taps >= 1 && threads >= 256 && threads % 256 == 0,
+              "taps must be positive and threads a positive multiple of 256");
+  if (int r = select_device(s)) return r;
+  LSH_CHECK_HIP(lsh::launch_probe_gather(window, width, height, taps, threads, out, s->stream),
+                "gather probe launch");
+  return LSH_OK;
+}
+
 }  // extern "C"

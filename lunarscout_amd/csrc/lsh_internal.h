@@ -132,6 +132,9 @@ struct LightArgs {
 
 hipError_t launch_lighting(const LightArgs& args, hipStream_t stream);
 hipError_t launch_march(const MarchArgs& args, hipStream_t stream);
+hipError_t launch_probe_copy(const float* src, float* dst, int64_t bytes, hipStream_t stream);
+hipError_t launch_probe_gather(const float* window, int32_t width, int32_t height, int32_t taps,
+                               int32_t threads, float* out, hipStream_t stream);
 
 // Fills a MarchDem from a registered DEM (float32 parameters the kernel reads).
 MarchDem march_dem(const Dem& dem);
