@@ -186,6 +186,25 @@ int lsh_psr_reduce(const float* elevation, int32_t width, int32_t height,
  * half_circle float32[16] and max_photons. */
 int lsh_sun_disk(double sun_half_angle_deg, float* half_circle, float* max_photons);
 
+/* .cbin horizon tile codec (host only; no device is touched).
+ * lsh_cbin_encode: the C# HorizonCompressor encoder (file_format.py:39-73,
+ * _write_compressed :128-133): degrees float32 [pixels][azimuths] (host, no NaN)
+ * -> payload (host, capacity bytes; pixels * (2 * azimuths + 2) always
+ * suffices): per pixel a little-endian u16 block length and the block.
+ * *payload_bytes receives the payload size. threads <= 0 uses every core.
+ * lsh_cbin_decode: the decoder with its structural validation
+ * (_decode_compressed_payload_into_python, file_format.py:135-182); LSH_EINVAL
+ * names the first structural error. degrees: float32 [pixels][azimuths].
+ * lsh_cbin_complete: HorizonTileStore.is_complete for a .cbin payload
+ * (file_format.py:335-356): *complete = 1 when every block length is in
+ * [1, 2 * azimuths], every block is present and nothing follows the last. */
+int lsh_cbin_encode(const float* degrees, int32_t pixels, int32_t azimuths, uint8_t* payload,
+                    int64_t capacity, int64_t* payload_bytes, int32_t threads);
+int lsh_cbin_decode(const uint8_t* payload, int64_t bytes, int32_t pixels, int32_t azimuths,
+                    float* degrees, int32_t threads);
+int lsh_cbin_complete(const uint8_t* payload, int64_t bytes, int32_t pixels, int32_t azimuths,
+                      int32_t* complete);
+
 /* Measurement probes (no counterpart in the reference; they calibrate the
  * roofline of the march on the running device).
  * lsh_probe_copy: device-to-device copy of `bytes` (a multiple of 16) by a

@@ -74,6 +74,9 @@ SIGNATURES = {
                            _f64, _f64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "lsh_psr_reduce": [_vp, _i32, _i32, _vp, _vp, _vp, _i32, _i32, _vp, ctypes.POINTER(_i32)],
     "lsh_sun_disk": [_f64, _vp, _vp],
+    "lsh_cbin_encode": [_vp, _i32, _i32, _vp, _i64, ctypes.POINTER(_i64), _i32],
+    "lsh_cbin_decode": [_vp, _i64, _i32, _i32, _vp, _i32],
+    "lsh_cbin_complete": [_vp, _i64, _i32, _i32, ctypes.POINTER(_i32)],
     "lsh_probe_copy": [_vp, _vp, _vp, ctypes.c_size_t],
     "lsh_probe_gather": [_vp, _vp, _i32, _i32, _i32, _i32, _vp],
     "lsh_event_create": [_vp, ctypes.POINTER(_vp)],

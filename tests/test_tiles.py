@@ -1,0 +1,154 @@
+"""The product .cbin codec and tile store (host code of the HIP library; no GPU).
+
+The codec is checked byte for byte against the oracle's restatement of the
+reference encoder (file_format.py:39-73), which is itself pinned by the round
+trip and quantisation bound in test_oracle_fixtures.py, and here additionally
+against hand-computed byte strings for each delta class, so that a dropped
+branch or an off-by-one at a class boundary (62/63, -64/-65, the +-16384
+clamp) fails on its own.
+"""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from lunarscout_amd import tiles
+
+
+def _random_tile(seed, pixels=512, jump_deg=30.0, infinities=True, rounding_column=True):
+    rng = np.random.default_rng(seed)
+    base = rng.normal(5.0, 20.0, (pixels, 1)).astype(np.float32)
+    walk = np.cumsum(rng.normal(0, 0.05, (pixels, 1440)), axis=1).astype(np.float32)
+    deg = base + walk
+    jumps = rng.random(deg.shape) < 0.01
+    deg[jumps] += rng.normal(0, jump_deg, jumps.sum()).astype(np.float32)
+    if infinities:
+        deg[rng.random(deg.shape) < 0.001] = -np.inf
+        deg[rng.random(deg.shape) < 0.001] = np.inf
+    if rounding_column:
+        deg[:, 7] = np.float32(0.5) / np.float32(655.34)  # near the half-way rounding point
+    return deg.astype(np.float32)
+
+
+def _quantum():
+    return np.float32(50.0) / np.float32(32767.0)
+
+
+def test_encoder_matches_the_oracle_byte_for_byte():
+    for seed in range(3):
+        deg = _random_tile(seed)
+        assert tiles.encode_cbin(deg) == oracle.cbin_payload(deg)
+        assert tiles.encode_cbin(deg, threads=1) == tiles.encode_cbin(deg, threads=7)
+
+
+def test_delta_classes_encode_to_known_bytes():
+    q = float(_quantum())
+    # quantized sequence 0, 62, 125 (d=63), 61 (d=-64), -4 (d=-65), 16379 (d=16383), -32767 clamp
+    quantized = [0, 62, 125, 61, -4, 16379, -5]
+    values = np.array([v * q for v in quantized] + [0.0] * (1440 - len(quantized)), np.float32)
+    payload = tiles.encode_cbin(values[None, :])
+    block = payload[2:]
+    assert int.from_bytes(payload[:2], "little") == len(block)
+    expected = bytes([0x00, 0x00,            # first sample 0, big endian
+                      62,                    # 0 <= d < 63: one byte
+                      0x80, 63,              # d = 63: two bytes
+                      (-64) & 0x7F,          # -64 <= d < 0: one byte
+                      0xFF, (-65) & 0xFF,    # d = -65: two bytes
+                      0xBF, 0xFF,            # d = 16383: two bytes
+                      0xC0, 0x00])           # d = -16384 (the lower clamp bound): two bytes
+    assert block[: len(expected)] == expected
+    back = tiles.decode_cbin(payload, pixels=1)
+    np.testing.assert_array_equal(back[0, :6], np.float32(quantized[:6]) * _quantum())
+
+
+def test_round_trip_error_is_within_half_a_quantum_inside_the_range():
+    deg = np.clip(_random_tile(5, jump_deg=4.0, infinities=False, rounding_column=False), -49.9, 49.9)
+    back = tiles.decode_cbin(tiles.encode_cbin(deg), pixels=deg.shape[0])
+    err = np.abs(back.astype(np.float64) - deg)
+    assert err.max() <= 0.5 * float(_quantum()) + 8e-6  # + float32 rounding near 50 deg
+    np.testing.assert_array_equal(back, oracle.cbin_decode(tiles.encode_cbin(deg), deg.shape[0]))
+
+
+def test_decoder_rejects_malformed_payloads():
+    deg = _random_tile(1, pixels=16)
+    payload = tiles.encode_cbin(deg)
+    tiles.decode_cbin(payload, pixels=16)
+    with pytest.raises(ValueError, match="trailing"):
+        tiles.decode_cbin(payload + b"\x00", pixels=16)
+    with pytest.raises(ValueError):
+        tiles.decode_cbin(payload[:-1], pixels=16)
+    bad = bytearray(payload)
+    bad[0:2] = (1).to_bytes(2, "little")  # a 1-byte block cannot hold the first sample
+    with pytest.raises(ValueError, match="length"):
+        tiles.decode_cbin(bytes(bad), pixels=16)
+    assert tiles.cbin_is_complete(payload, pixels=16)
+    assert not tiles.cbin_is_complete(payload[:-1], pixels=16)
+    assert not tiles.cbin_is_complete(payload + b"\x00", pixels=16)
+
+
+def test_encoder_rejects_nan():
+    deg = _random_tile(2, pixels=4)
+    deg[1, 5] = np.nan
+    with pytest.raises(ValueError, match="NaN"):
+        tiles.encode_cbin(deg)
+
+
+def test_store_names_lookup_order_and_resume_rule(tmp_path):
+    store = tiles.HorizonTileStore(tmp_path)
+    assert store.build_file_name(256, 384, 1.25, compress=True) == "horizon_00256_00384_012.cbin"
+    assert store.build_path(0, 128, 0.0, compress=False) == tmp_path / "00000" / "horizon_00000_00128_000.bin"
+    with pytest.raises(ValueError):
+        store.build_file_name(0, 100000, 0.0, compress=True)
+    with pytest.raises(ValueError):
+        store.build_file_name(0, 0, 100.0, compress=True)
+    cands = store.candidate_paths(128, 0, 0.0)
+    assert [p.name for p in cands] == ["horizon_00128_00000_000.cbin", "horizon_00128_00000_000.bin"] * 2
+    assert cands[2].parent == tmp_path
+    assert store.find_existing_path(128, 0, 0.0) is None
+    # an incomplete .bin is not resumable; a complete flat legacy .bin is
+    flat = tmp_path / "horizon_00128_00000_000.bin"
+    flat.write_bytes(b"\x00" * 100)
+    assert store.find_existing_path(128, 0, 0.0) is None
+    assert store.find_existing_path(128, 0, 0.0, require_complete=False) == flat
+    flat.write_bytes(b"\x00" * (tiles.RAW_FILE_BYTES + 28))
+    assert store.find_existing_path(128, 0, 0.0) == flat
+
+
+def test_store_writes_pads_edges_and_reads_back(tmp_path):
+    store = tiles.HorizonTileStore(tmp_path)
+    rng = np.random.default_rng(4)
+    azimuth = np.radians(np.arange(1440) * 0.25)
+    valid = (rng.uniform(-10, 40, (100 * 77, 1)) + 5.0 * np.sin(azimuth)[None, :]).astype(np.float32)
+    for compress in (True, False):
+        path = store.write(0, 128, 0.0, valid, compress=compress, valid_width=77, valid_height=100)
+        assert path.is_file() and store.is_complete(path)
+        assert not [p for p in path.parent.iterdir() if ".tmp" in p.name]
+        back = store.read(0, 128, 0.0) if compress else tiles.read_horizon_tile(path)
+        assert back.shape == (128, 128, 1440)
+        assert np.all(back[100:] == np.float32(-50.0)) and np.all(back[:, 77:] == np.float32(-50.0))
+        region = back[:100, :77].reshape(-1, 1440)
+        if compress:
+            assert np.max(np.abs(region - valid)) <= 0.5 * float(_quantum()) + 8e-6
+        else:
+            np.testing.assert_array_equal(region, valid)
+    assert store.find_existing_path(0, 128, 0.0).suffix == ".cbin"
+    with pytest.raises(ValueError, match="NaN"):
+        store.write(0, 0, 0.0, np.full((16384, 1440), np.nan, np.float32), compress=False)
+    assert not (tmp_path / "00000" / "horizon_00000_00000_000.bin").exists()
+
+
+def test_read_into_output_buffer(tmp_path):
+    store = tiles.HorizonTileStore(tmp_path)
+    deg = np.tile(np.linspace(-5, 5, 1440, dtype=np.float32), (16384, 1))
+    path = store.write(128, 128, 2.5, deg, compress=True)
+    out = np.empty((128, 128, 1440), np.float32)
+    values, timings = tiles.read_horizon_tile_with_timings(path, output=out)
+    assert np.shares_memory(values, out)
+    assert timings.decompression_seconds >= 0.0
+    with pytest.raises(ValueError):
+        tiles.read_horizon_tile_with_timings(path, output=np.empty((128, 128, 1440), np.float64))
+    os.remove(path)
