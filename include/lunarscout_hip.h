@@ -174,6 +174,23 @@ int lsh_lighting_tiles(lsh_session* session, int32_t dem_id, const lsh_tile* til
                        float* earth_margin, float* earth_visible, uint8_t* psr,
                        float* psr_margin);
 
+/* The same lighting on host threads: the "cpu" backend of the lighting products
+ * (LightmapCpuSession, lightmap_cpu.py:193-237; compute_psr_patch_reference,
+ * psr.py:94-140). Every buffer is host memory: elevation float32
+ * [height][width], geo_transform / projection float64[6] (as lsh_dem_create),
+ * the other arguments and outputs as lsh_lighting_tiles. threads <= 0 uses
+ * every core. */
+int lsh_lighting_tiles_host(const float* elevation, int32_t width, int32_t height,
+                            const double* geo_transform, const double* projection,
+                            const lsh_tile* tiles, int32_t tile_count, const float* horizons,
+                            int32_t azimuth_count, int32_t azimuth_major, const double* sun,
+                            int32_t sun_count, const double* earth, int32_t earth_count,
+                            const double* psr_vectors, int32_t psr_count,
+                            double sun_half_angle_deg, double psr_diameter_deg, int32_t flags,
+                            int32_t threads, uint8_t* sun_bytes, float* sun_fraction,
+                            float* sun_mean, float* earth_margin, float* earth_visible,
+                            uint8_t* psr, float* psr_margin);
+
 /* Five-viewpoint, per-azimuth-bin maximum-elevation reduction of Sun vectors
  * for PSR (reduce_sun_vectors_for_psr, psr.py:61-91). Host-only float64.
  * elevation: host float32[height][width]; vectors float64[count][3];

@@ -72,6 +72,9 @@ SIGNATURES = {
     "lsh_horizon_tile_full": [_vp, _vp, _i32, _i32, _i32, _i32, _f64, _f32, _i32, _vp, _vp],
     "lsh_lighting_tiles": [_vp, _i32, _vp, _i32, _vp, _i32, _i32, _vp, _i32, _vp, _i32, _vp, _i32,
                            _f64, _f64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "lsh_lighting_tiles_host": [_vp, _i32, _i32, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _vp, _i32,
+                                _vp, _i32, _vp, _i32, _f64, _f64, _i32, _i32, _vp, _vp, _vp, _vp,
+                                _vp, _vp, _vp],
     "lsh_psr_reduce": [_vp, _i32, _i32, _vp, _vp, _vp, _i32, _i32, _vp, ctypes.POINTER(_i32)],
     "lsh_sun_disk": [_f64, _vp, _vp],
     "lsh_cbin_encode": [_vp, _i32, _i32, _vp, _i64, ctypes.POINTER(_i64), _i32],

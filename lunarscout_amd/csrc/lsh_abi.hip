@@ -500,17 +500,15 @@ int lsh_sun_disk(double sun_half_angle_deg, float* half_circle, float* max_photo
   return LSH_OK;
 }
 
-int lsh_lighting_tiles(lsh_session* session, int32_t dem_id, const lsh_tile* tiles,
-                       int32_t tile_count, const float* horizons, int32_t azimuth_count,
-                       int32_t azimuth_major, const double* sun, int32_t sun_count,
-                       const double* earth, int32_t earth_count, const double* psr_vectors,
-                       int32_t psr_count, double sun_half_angle_deg, double psr_diameter_deg,
-                       int32_t flags, uint8_t* sun_bytes, float* sun_fraction, float* sun_mean,
-                       float* earth_margin, float* earth_visible, uint8_t* psr,
-                       float* psr_margin) {
-  Session* s = reinterpret_cast<Session*>(session);
-  Dem* d = find_dem(s, dem_id);
-  LSH_REQUIRE(d, "unknown DEM id");
+// Validates a lighting call and fills everything of LightArgs but the DEM and tiles.
+static int light_args(const lsh_tile* tiles, int32_t tile_count, int32_t dem_width,
+                      int32_t dem_height, const float* horizons, int32_t azimuth_count,
+                      int32_t azimuth_major, const double* sun, int32_t sun_count,
+                      const double* earth, int32_t earth_count, const double* psr_vectors,
+                      int32_t psr_count, double sun_half_angle_deg, double psr_diameter_deg,
+                      int32_t flags, uint8_t* sun_bytes, float* sun_fraction, float* sun_mean,
+                      float* earth_margin, float* earth_visible, uint8_t* psr, float* psr_margin,
+                      lsh::LightArgs* out) {
   LSH_REQUIRE(tiles && tile_count > 0 && tile_count <= 65535, "tile_count must be 1..65535");
   LSH_REQUIRE(horizons && azimuth_count >= 2, "horizons and azimuth_count >= 2 are required");
   LSH_REQUIRE(sun_count >= 0 && earth_count >= 0 && psr_count >= 0, "counts must be >= 0");
@@ -522,14 +520,19 @@ int lsh_lighting_tiles(lsh_session* session, int32_t dem_id, const lsh_tile* til
               "disk sizes must be positive");
   for (int i = 0; i < tile_count; ++i)
     LSH_REQUIRE(tiles[i].tile_column >= 0 && tiles[i].tile_row >= 0 &&
-                    tiles[i].tile_column < d->width && tiles[i].tile_row < d->height,
+                    tiles[i].tile_column < dem_width && tiles[i].tile_row < dem_height,
                 "tile origin lies outside the DEM");
-  lsh::LightArgs a{};
-  a.elevation = d->level0;
-  a.dem_width = d->width;
-  a.dem_height = d->height;
-  std::memcpy(a.geo_transform, d->geo_transform, sizeof(a.geo_transform));
-  std::memcpy(a.projection, d->projection, sizeof(a.projection));
+  LSH_REQUIRE(!(flags & LSH_LIGHT_SUN_BYTES) || sun_bytes, "sun_bytes output missing");
+  LSH_REQUIRE(!(flags & LSH_LIGHT_SUN_FRACTION) || sun_fraction, "sun_fraction output missing");
+  LSH_REQUIRE(!(flags & LSH_LIGHT_SUN_MEAN) || sun_mean, "sun_mean output missing");
+  LSH_REQUIRE(!(flags & LSH_LIGHT_EARTH_MARGIN) || earth_margin, "earth_margin output missing");
+  LSH_REQUIRE(!(flags & LSH_LIGHT_EARTH_VISIBLE) || earth_visible, "earth_visible output missing");
+  LSH_REQUIRE(!(flags & LSH_LIGHT_PSR) || psr, "psr output missing");
+  LSH_REQUIRE(!(flags & LSH_LIGHT_PSR_MARGIN) || psr_margin, "psr_margin output missing");
+  lsh::LightArgs& a = *out;
+  a = lsh::LightArgs{};
+  a.dem_width = dem_width;
+  a.dem_height = dem_height;
   a.tile_count = tile_count;
   a.horizons = horizons;
   a.N = azimuth_count;
@@ -550,17 +553,59 @@ int lsh_lighting_tiles(lsh_session* session, int32_t dem_id, const lsh_tile* til
   a.earth_visible = (flags & LSH_LIGHT_EARTH_VISIBLE) ? earth_visible : nullptr;
   a.psr = (flags & LSH_LIGHT_PSR) ? psr : nullptr;
   a.psr_margin = (flags & LSH_LIGHT_PSR_MARGIN) ? psr_margin : nullptr;
-  LSH_REQUIRE(!(flags & LSH_LIGHT_SUN_BYTES) || sun_bytes, "sun_bytes output missing");
-  LSH_REQUIRE(!(flags & LSH_LIGHT_SUN_FRACTION) || sun_fraction, "sun_fraction output missing");
-  LSH_REQUIRE(!(flags & LSH_LIGHT_SUN_MEAN) || sun_mean, "sun_mean output missing");
-  LSH_REQUIRE(!(flags & LSH_LIGHT_EARTH_MARGIN) || earth_margin, "earth_margin output missing");
-  LSH_REQUIRE(!(flags & LSH_LIGHT_EARTH_VISIBLE) || earth_visible, "earth_visible output missing");
-  LSH_REQUIRE(!(flags & LSH_LIGHT_PSR) || psr, "psr output missing");
-  LSH_REQUIRE(!(flags & LSH_LIGHT_PSR_MARGIN) || psr_margin, "psr_margin output missing");
+  return LSH_OK;
+}
+
+int lsh_lighting_tiles(lsh_session* session, int32_t dem_id, const lsh_tile* tiles,
+                       int32_t tile_count, const float* horizons, int32_t azimuth_count,
+                       int32_t azimuth_major, const double* sun, int32_t sun_count,
+                       const double* earth, int32_t earth_count, const double* psr_vectors,
+                       int32_t psr_count, double sun_half_angle_deg, double psr_diameter_deg,
+                       int32_t flags, uint8_t* sun_bytes, float* sun_fraction, float* sun_mean,
+                       float* earth_margin, float* earth_visible, uint8_t* psr,
+                       float* psr_margin) {
+  Session* s = reinterpret_cast<Session*>(session);
+  Dem* d = find_dem(s, dem_id);
+  LSH_REQUIRE(d, "unknown DEM id");
+  lsh::LightArgs a;
+  if (int r = light_args(tiles, tile_count, d->width, d->height, horizons, azimuth_count,
+                         azimuth_major, sun, sun_count, earth, earth_count, psr_vectors, psr_count,
+                         sun_half_angle_deg, psr_diameter_deg, flags, sun_bytes, sun_fraction,
+                         sun_mean, earth_margin, earth_visible, psr, psr_margin, &a))
+    return r;
+  a.elevation = d->level0;
+  std::memcpy(a.geo_transform, d->geo_transform, sizeof(a.geo_transform));
+  std::memcpy(a.projection, d->projection, sizeof(a.projection));
   if (int r = select_device(s)) return r;
   std::lock_guard<std::mutex> lock(s->mutex);
   if (int r = lsh::upload_tiles(s, tiles, tile_count, &a.tiles)) return r;
   LSH_CHECK_HIP(lsh::launch_lighting(a, s->stream), "lighting kernel launch");
+  return LSH_OK;
+}
+
+int lsh_lighting_tiles_host(const float* elevation, int32_t width, int32_t height,
+                            const double* geo_transform, const double* projection,
+                            const lsh_tile* tiles, int32_t tile_count, const float* horizons,
+                            int32_t azimuth_count, int32_t azimuth_major, const double* sun,
+                            int32_t sun_count, const double* earth, int32_t earth_count,
+                            const double* psr_vectors, int32_t psr_count,
+                            double sun_half_angle_deg, double psr_diameter_deg, int32_t flags,
+                            int32_t threads, uint8_t* sun_bytes, float* sun_fraction,
+                            float* sun_mean, float* earth_margin, float* earth_visible,
+                            uint8_t* psr, float* psr_margin) {
+  LSH_REQUIRE(elevation && geo_transform && projection && width > 0 && height > 0,
+              "DEM arguments are invalid");
+  lsh::LightArgs a;
+  if (int r = light_args(tiles, tile_count, width, height, horizons, azimuth_count, azimuth_major,
+                         sun, sun_count, earth, earth_count, psr_vectors, psr_count,
+                         sun_half_angle_deg, psr_diameter_deg, flags, sun_bytes, sun_fraction,
+                         sun_mean, earth_margin, earth_visible, psr, psr_margin, &a))
+    return r;
+  a.elevation = elevation;
+  std::memcpy(a.geo_transform, geo_transform, sizeof(a.geo_transform));
+  std::memcpy(a.projection, projection, sizeof(a.projection));
+  a.tiles = tiles;
+  lsh::run_lighting_host(a, threads);
   return LSH_OK;
 }
 

@@ -131,6 +131,7 @@ struct LightArgs {
 };
 
 hipError_t launch_lighting(const LightArgs& args, hipStream_t stream);
+void run_lighting_host(const LightArgs& args, int threads);
 hipError_t launch_march(const MarchArgs& args, hipStream_t stream);
 hipError_t launch_probe_copy(const float* src, float* dst, int64_t bytes, hipStream_t stream);
 hipError_t launch_probe_gather(const float* window, int32_t width, int32_t height, int32_t taps,

@@ -1,15 +1,7 @@
 // lsh_lighting.hip -- lighting from stored horizons: sun fraction (finite disk),
 // Earth margin / visibility and the PSR mask, in one streamed pass per pixel.
 //
-// Restates (paths under src/lunarscout/_numba_horizon/):
-//   * the pixel frame _pixel_frame and _azimuth_elevation_deg (psr.py:29-58),
-//     evaluated in float64 (the reference kernels round the frame to float32);
-//   * _sun_fraction_reference / BuilderSunFraction (lightmap.py:32-73,
-//     lightmap_cuda.py:146-182) in float32, with the slice table of lightmap.py:15-29;
-//   * the body-centre margin el - lerp(horizon) (lightmap_cpu.py:121-143,
-//     lightmap.py:116-120) and the PSR upper-limb test (psr.py:94-140,
-//     psr_cuda.py:171-186).
-//
+// The per-pixel arithmetic is lsh_lighting_core.h (shared with the host path).
 // One thread per pixel walks the whole time series; all threads of a workgroup
 // read the same Moon-ME vector (a uniform, scalar load) and each reads only its
 // own horizon row, so HBM traffic is the horizons once plus the outputs. The
@@ -17,206 +9,20 @@
 // azimuth-major [azimuth][pixel], in which case the 64 lanes of a wavefront
 // read one contiguous 256-byte run per tap.
 
-#include <cmath>
+#include <thread>
+#include <vector>
 
-#include "lsh_internal.h"
+#include "lsh_lighting_core.h"
 
 namespace lsh {
 namespace {
-
-constexpr double kPi = 3.14159265358979323846;
-constexpr int kPatchPixels = 128 * 128;
-
-struct Frame {
-  double rot[3][3];  // rot[k][j]: component k of basis j (east, north, up)
-  double tr[3];
-};
-
-// _pixel_frame (psr.py:29-47) in float64.
-__device__ Frame pixel_frame(const LightArgs& a, int row, int column) {
-  const double* t = a.geo_transform;
-  const double* p = a.projection;
-  double x = t[0] + t[1] * column + t[2] * row;
-  double y = t[3] + t[4] * column + t[5] * row;
-  double xp = x - p[4], yp = y - p[5];
-  double rho = hypot(xp, yp);
-  double lat, lon;
-  if (rho < 1e-9) {
-    lat = p[1];
-    lon = p[2];
-  } else {
-    double c = 2.0 * atan2(rho, 2.0 * p[3] * p[0]);
-    double sc, cc, so, co;
-    sincos(c, &sc, &cc);
-    sincos(p[1], &so, &co);
-    lat = asin(cc * so + yp * sc * co / rho);
-    lon = p[2] + atan2(xp * sc, rho * co * cc - yp * so * sc);
-  }
-  double sl, cl, sn, cn;
-  sincos(lat, &sl, &cl);
-  sincos(lon, &sn, &cn);
-  double up[3] = {cl * cn, cl * sn, sl};
-  double east[3] = {-sn, cn, 0.0};
-  double north[3] = {-sl * cn, -sl * sn, cl};
-  double radius = p[0] + static_cast<double>(a.elevation[static_cast<int64_t>(row) * a.dem_width + column]);
-  double obs[3] = {radius * up[0], radius * up[1], radius * up[2]};
-  Frame f;
-  for (int k = 0; k < 3; ++k) {
-    f.rot[k][0] = east[k];
-    f.rot[k][1] = north[k];
-    f.rot[k][2] = up[k];
-  }
-  for (int j = 0; j < 3; ++j) {
-    double acc = 0.0;
-    for (int k = 0; k < 3; ++k) acc += obs[k] * f.rot[k][j];
-    f.tr[j] = -acc;
-  }
-  return f;
-}
-
-// _azimuth_elevation_deg (psr.py:50-58) in float64.
-__device__ inline void azimuth_elevation(const Frame& f, const double* v, double& az, double& el) {
-  double e[3];
-  for (int j = 0; j < 3; ++j)
-    e[j] = v[0] * f.rot[0][j] + v[1] * f.rot[1][j] + v[2] * f.rot[2][j] + f.tr[j];
-  double a = atan2(e[0], e[1]) * (180.0 / kPi);
-  a = fmod(a, 360.0);
-  if (a < 0.0) a += 360.0;
-  az = a;
-  el = atan2(e[2], hypot(e[0], e[1])) * (180.0 / kPi);
-}
-
-template <bool kAzimuthMajor>
-struct HorizonRow {
-  const float* base;
-  int N;
-  __device__ inline float operator[](int bin) const {
-    return kAzimuthMajor ? base[static_cast<int64_t>(bin) * kPatchPixels] : base[bin];
-  }
-};
-
-// Interpolated horizon at an azimuth (psr.py:127-134), float64 position.
-template <class Row>
-__device__ inline double horizon_at(const Row& hz, int N, double az) {
-  double pos = az * (N / 360.0);
-  double fl = floor(pos);
-  int lower = static_cast<int>(fl) % N;
-  if (lower < 0) lower += N;
-  int upper = lower + 1 == N ? 0 : lower + 1;
-  double f = pos - fl;
-  double lo = hz[lower];
-  return lo + f * (static_cast<double>(hz[upper]) - lo);
-}
-
-// BuilderSunFraction (lightmap.py:32-73) in float32.
-template <class Row>
-__device__ inline float sun_fraction(const Row& hz, int N, float az_deg, float el_deg,
-                                     const LightArgs& a) {
-  float left_pos = (az_deg - a.sun_half_angle - 0.125f) * 4.0f;
-  int left = static_cast<int>(left_pos);  // truncation toward zero
-  float frac = left_pos - static_cast<float>(left);
-  if (left < 0) left += N;
-  else if (left >= N) left -= N;
-  int right = left + 1;
-  if (right >= N) right = 0;
-  float le = hz[left], re = hz[right];
-  float delta = re - le;
-  float photons = 0.0f;
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const float col = a.half_circle[i];
-    const float horizon = frac * delta + le;
-    const float top = el_deg + col;
-    if (horizon < top) {
-      float ad = top - horizon;
-      const float ch = col + col;
-      if (ad > ch) ad = ch;
-      photons = photons + ad;
-    }
-    frac = frac + 0.135f;
-    if (frac >= 1.0f) {
-      right += 1;
-      if (right >= N) right = 0;
-      le = re;
-      re = hz[right];
-      delta = re - le;
-      frac = frac - 1.0f;
-    }
-  }
-  return photons / a.max_photons;
-}
 
 template <bool kAzimuthMajor>
 __global__ void __launch_bounds__(256) lighting_kernel(LightArgs a) {
   const int tile = blockIdx.y;
   const int pixel = blockIdx.x * blockDim.x + threadIdx.x;
   if (pixel >= kPatchPixels) return;
-  const lsh_tile origin = a.tiles[tile];
-  const int line = pixel / 128, sample = pixel % 128;
-  const int row = origin.tile_row + line, column = origin.tile_column + sample;
-  const int64_t tile_pixels = static_cast<int64_t>(tile) * kPatchPixels;
-  const bool inside = row < a.dem_height && column < a.dem_width;
-  if (!inside) {  // edge padding (psr_cuda.py:74-76)
-    if (a.sun_bytes)
-      for (int t = 0; t < a.sun_count; ++t)
-        a.sun_bytes[(static_cast<int64_t>(tile) * a.sun_count + t) * kPatchPixels + pixel] = 0;
-    if (a.sun_fraction)
-      for (int t = 0; t < a.sun_count; ++t)
-        a.sun_fraction[(static_cast<int64_t>(tile) * a.sun_count + t) * kPatchPixels + pixel] = 0.0f;
-    if (a.earth_margin)
-      for (int t = 0; t < a.earth_count; ++t)
-        a.earth_margin[(static_cast<int64_t>(tile) * a.earth_count + t) * kPatchPixels + pixel] = -INFINITY;
-    if (a.sun_mean) a.sun_mean[tile_pixels + pixel] = 0.0f;
-    if (a.earth_visible) a.earth_visible[tile_pixels + pixel] = 0.0f;
-    if (a.psr) a.psr[tile_pixels + pixel] = 0;
-    if (a.psr_margin) a.psr_margin[tile_pixels + pixel] = -INFINITY;
-    return;
-  }
-  const int N = a.N;
-  HorizonRow<kAzimuthMajor> hz;
-  hz.N = N;
-  hz.base = kAzimuthMajor ? a.horizons + static_cast<int64_t>(tile) * kPatchPixels * N + pixel
-                          : a.horizons + (tile_pixels + pixel) * N;
-  const Frame f = pixel_frame(a, row, column);
-
-  if (a.sun_count > 0 && (a.sun_bytes || a.sun_fraction || a.sun_mean)) {
-    float acc = 0.0f;
-    for (int t = 0; t < a.sun_count; ++t) {
-      double az, el;
-      azimuth_elevation(f, a.sun + 3 * t, az, el);
-      float fr = sun_fraction(hz, N, static_cast<float>(az), static_cast<float>(el), a);
-      acc += fr;
-      int64_t o = (static_cast<int64_t>(tile) * a.sun_count + t) * kPatchPixels + pixel;
-      if (a.sun_bytes) a.sun_bytes[o] = static_cast<uint8_t>(255.0f * fr);
-      if (a.sun_fraction) a.sun_fraction[o] = fr;
-    }
-    if (a.sun_mean) a.sun_mean[tile_pixels + pixel] = acc / static_cast<float>(a.sun_count);
-  }
-  if (a.earth_count > 0 && (a.earth_margin || a.earth_visible)) {
-    int visible = 0;
-    for (int t = 0; t < a.earth_count; ++t) {
-      double az, el;
-      azimuth_elevation(f, a.earth + 3 * t, az, el);
-      double m = el - horizon_at(hz, N, az);
-      visible += m >= 0.0;
-      if (a.earth_margin)
-        a.earth_margin[(static_cast<int64_t>(tile) * a.earth_count + t) * kPatchPixels + pixel] =
-            static_cast<float>(m);
-    }
-    if (a.earth_visible)
-      a.earth_visible[tile_pixels + pixel] = static_cast<float>(visible) / a.earth_count;
-  }
-  if (a.psr_count > 0 && (a.psr || a.psr_margin)) {
-    double best = -1e300;
-    for (int t = 0; t < a.psr_count; ++t) {
-      double az, el;
-      azimuth_elevation(f, a.psr_vectors + 3 * t, az, el);
-      double m = el - horizon_at(hz, N, az);
-      best = m > best ? m : best;
-    }
-    if (a.psr) a.psr[tile_pixels + pixel] = best > a.psr_threshold ? 0 : 255;
-    if (a.psr_margin) a.psr_margin[tile_pixels + pixel] = static_cast<float>(best);
-  }
+  light_pixel<kAzimuthMajor>(a, tile, pixel, a.tiles[tile]);
 }
 
 }  // namespace
@@ -228,6 +34,32 @@ hipError_t launch_lighting(const LightArgs& a, hipStream_t stream) {
   else
     hipLaunchKernelGGL(lighting_kernel<false>, grid, block, 0, stream, a);
   return hipGetLastError();
+}
+
+}  // namespace lsh
+
+namespace lsh {
+
+// The host path: the same per-pixel arithmetic over (tile, pixel) ranges on
+// `threads` host threads (the "cpu" backend of the lighting products).
+void run_lighting_host(const LightArgs& a, int threads) {
+  const int64_t items = static_cast<int64_t>(a.tile_count) * kPatchPixels;
+  int n = threads > 0 ? threads : static_cast<int>(std::thread::hardware_concurrency());
+  n = n < 1 ? 1 : (n > 256 ? 256 : n);
+  auto body = [&a, items, n](int t) {
+    const int64_t begin = items * t / n, end = items * (t + 1) / n;
+    for (int64_t i = begin; i < end; ++i) {
+      const int tile = static_cast<int>(i / kPatchPixels), pixel = static_cast<int>(i % kPatchPixels);
+      if (a.azimuth_major)
+        light_pixel<true>(a, tile, pixel, a.tiles[tile]);
+      else
+        light_pixel<false>(a, tile, pixel, a.tiles[tile]);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < n; ++t) pool.emplace_back(body, t);
+  body(0);
+  for (auto& th : pool) th.join();
 }
 
 }  // namespace lsh

@@ -1,0 +1,220 @@
+// lsh_lighting_core.h -- per-pixel lighting from a stored horizon row, shared by
+// the HIP kernel and the multi-threaded host path (the "cpu" backend).
+//
+// Restates (paths under src/lunarscout/_numba_horizon/):
+//   * _pixel_frame and _azimuth_elevation_deg (psr.py:29-58) in float64;
+//   * _sun_fraction_reference / BuilderSunFraction (lightmap.py:32-73,
+//     lightmap_cuda.py:146-182) in float32, with the slice table of lightmap.py:15-29;
+//   * the body-centre margin el - lerp(horizon) (lightmap_cpu.py:121-143,
+//     lightmap.py:116-120) and the PSR upper-limb test (psr.py:94-140,
+//     psr_cuda.py:171-186).
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+
+#include "lsh_internal.h"
+
+#if defined(__HIPCC__)
+#define LSH_LHD __host__ __device__ inline
+#else
+#define LSH_LHD inline
+#endif
+
+namespace lsh {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kPatchPixels = 128 * 128;
+
+struct Frame {
+  double rot[3][3];  // rot[k][j]: component k of basis j (east, north, up)
+  double tr[3];
+};
+
+// _pixel_frame (psr.py:29-47) in float64.
+LSH_LHD Frame pixel_frame(const LightArgs& a, int row, int column) {
+  const double* t = a.geo_transform;
+  const double* p = a.projection;
+  double x = t[0] + t[1] * column + t[2] * row;
+  double y = t[3] + t[4] * column + t[5] * row;
+  double xp = x - p[4], yp = y - p[5];
+  double rho = hypot(xp, yp);
+  double lat, lon;
+  if (rho < 1e-9) {
+    lat = p[1];
+    lon = p[2];
+  } else {
+    double c = 2.0 * atan2(rho, 2.0 * p[3] * p[0]);
+    double sc, cc, so, co;
+    sincos(c, &sc, &cc);
+    sincos(p[1], &so, &co);
+    lat = asin(cc * so + yp * sc * co / rho);
+    lon = p[2] + atan2(xp * sc, rho * co * cc - yp * so * sc);
+  }
+  double sl, cl, sn, cn;
+  sincos(lat, &sl, &cl);
+  sincos(lon, &sn, &cn);
+  double up[3] = {cl * cn, cl * sn, sl};
+  double east[3] = {-sn, cn, 0.0};
+  double north[3] = {-sl * cn, -sl * sn, cl};
+  double radius = p[0] + static_cast<double>(a.elevation[static_cast<int64_t>(row) * a.dem_width + column]);
+  double obs[3] = {radius * up[0], radius * up[1], radius * up[2]};
+  Frame f;
+  for (int k = 0; k < 3; ++k) {
+    f.rot[k][0] = east[k];
+    f.rot[k][1] = north[k];
+    f.rot[k][2] = up[k];
+  }
+  for (int j = 0; j < 3; ++j) {
+    double acc = 0.0;
+    for (int k = 0; k < 3; ++k) acc += obs[k] * f.rot[k][j];
+    f.tr[j] = -acc;
+  }
+  return f;
+}
+
+// _azimuth_elevation_deg (psr.py:50-58) in float64.
+LSH_LHD void azimuth_elevation(const Frame& f, const double* v, double& az, double& el) {
+  double e[3];
+  for (int j = 0; j < 3; ++j)
+    e[j] = v[0] * f.rot[0][j] + v[1] * f.rot[1][j] + v[2] * f.rot[2][j] + f.tr[j];
+  double a = atan2(e[0], e[1]) * (180.0 / kPi);
+  a = fmod(a, 360.0);
+  if (a < 0.0) a += 360.0;
+  az = a;
+  el = atan2(e[2], hypot(e[0], e[1])) * (180.0 / kPi);
+}
+
+template <bool kAzimuthMajor>
+struct HorizonRow {
+  const float* base;
+  int N;
+  LSH_LHD float operator[](int bin) const {
+    return kAzimuthMajor ? base[static_cast<int64_t>(bin) * kPatchPixels] : base[bin];
+  }
+};
+
+// Interpolated horizon at an azimuth (psr.py:127-134), float64 position.
+template <class Row>
+LSH_LHD double horizon_at(const Row& hz, int N, double az) {
+  double pos = az * (N / 360.0);
+  double fl = floor(pos);
+  int lower = static_cast<int>(fl) % N;
+  if (lower < 0) lower += N;
+  int upper = lower + 1 == N ? 0 : lower + 1;
+  double f = pos - fl;
+  double lo = hz[lower];
+  return lo + f * (static_cast<double>(hz[upper]) - lo);
+}
+
+// BuilderSunFraction (lightmap.py:32-73) in float32.
+template <class Row>
+LSH_LHD float sun_fraction(const Row& hz, int N, float az_deg, float el_deg,
+                                     const LightArgs& a) {
+  float left_pos = (az_deg - a.sun_half_angle - 0.125f) * 4.0f;
+  int left = static_cast<int>(left_pos);  // truncation toward zero
+  float frac = left_pos - static_cast<float>(left);
+  if (left < 0) left += N;
+  else if (left >= N) left -= N;
+  int right = left + 1;
+  if (right >= N) right = 0;
+  float le = hz[left], re = hz[right];
+  float delta = re - le;
+  float photons = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const float col = a.half_circle[i];
+    const float horizon = frac * delta + le;
+    const float top = el_deg + col;
+    if (horizon < top) {
+      float ad = top - horizon;
+      const float ch = col + col;
+      if (ad > ch) ad = ch;
+      photons = photons + ad;
+    }
+    frac = frac + 0.135f;
+    if (frac >= 1.0f) {
+      right += 1;
+      if (right >= N) right = 0;
+      le = re;
+      re = hz[right];
+      delta = re - le;
+      frac = frac - 1.0f;
+    }
+  }
+  return photons / a.max_photons;
+}
+
+
+// Every output of one pixel of one tile: the time series walked in order.
+template <bool kAzimuthMajor>
+LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origin) {
+  const int line = pixel / 128, sample = pixel % 128;
+  const int row = origin.tile_row + line, column = origin.tile_column + sample;
+  const int64_t tile_pixels = static_cast<int64_t>(tile) * kPatchPixels;
+  const bool inside = row < a.dem_height && column < a.dem_width;
+  if (!inside) {  // edge padding (psr_cuda.py:74-76)
+    if (a.sun_bytes)
+      for (int t = 0; t < a.sun_count; ++t)
+        a.sun_bytes[(static_cast<int64_t>(tile) * a.sun_count + t) * kPatchPixels + pixel] = 0;
+    if (a.sun_fraction)
+      for (int t = 0; t < a.sun_count; ++t)
+        a.sun_fraction[(static_cast<int64_t>(tile) * a.sun_count + t) * kPatchPixels + pixel] = 0.0f;
+    if (a.earth_margin)
+      for (int t = 0; t < a.earth_count; ++t)
+        a.earth_margin[(static_cast<int64_t>(tile) * a.earth_count + t) * kPatchPixels + pixel] =
+            -INFINITY;
+    if (a.sun_mean) a.sun_mean[tile_pixels + pixel] = 0.0f;
+    if (a.earth_visible) a.earth_visible[tile_pixels + pixel] = 0.0f;
+    if (a.psr) a.psr[tile_pixels + pixel] = 0;
+    if (a.psr_margin) a.psr_margin[tile_pixels + pixel] = -INFINITY;
+    return;
+  }
+  const int N = a.N;
+  HorizonRow<kAzimuthMajor> hz;
+  hz.N = N;
+  hz.base = kAzimuthMajor ? a.horizons + static_cast<int64_t>(tile) * kPatchPixels * N + pixel
+                          : a.horizons + (tile_pixels + pixel) * N;
+  const Frame f = pixel_frame(a, row, column);
+
+  if (a.sun_count > 0 && (a.sun_bytes || a.sun_fraction || a.sun_mean)) {
+    float acc = 0.0f;
+    for (int t = 0; t < a.sun_count; ++t) {
+      double az, el;
+      azimuth_elevation(f, a.sun + 3 * t, az, el);
+      float fr = sun_fraction(hz, N, static_cast<float>(az), static_cast<float>(el), a);
+      acc += fr;
+      int64_t o = (static_cast<int64_t>(tile) * a.sun_count + t) * kPatchPixels + pixel;
+      if (a.sun_bytes) a.sun_bytes[o] = static_cast<uint8_t>(255.0f * fr);
+      if (a.sun_fraction) a.sun_fraction[o] = fr;
+    }
+    if (a.sun_mean) a.sun_mean[tile_pixels + pixel] = acc / static_cast<float>(a.sun_count);
+  }
+  if (a.earth_count > 0 && (a.earth_margin || a.earth_visible)) {
+    int visible = 0;
+    for (int t = 0; t < a.earth_count; ++t) {
+      double az, el;
+      azimuth_elevation(f, a.earth + 3 * t, az, el);
+      double m = el - horizon_at(hz, N, az);
+      visible += m >= 0.0;
+      if (a.earth_margin)
+        a.earth_margin[(static_cast<int64_t>(tile) * a.earth_count + t) * kPatchPixels + pixel] =
+            static_cast<float>(m);
+    }
+    if (a.earth_visible)
+      a.earth_visible[tile_pixels + pixel] = static_cast<float>(visible) / a.earth_count;
+  }
+  if (a.psr_count > 0 && (a.psr || a.psr_margin)) {
+    double best = -1e300;
+    for (int t = 0; t < a.psr_count; ++t) {
+      double az, el;
+      azimuth_elevation(f, a.psr_vectors + 3 * t, az, el);
+      double m = el - horizon_at(hz, N, az);
+      best = m > best ? m : best;
+    }
+    if (a.psr) a.psr[tile_pixels + pixel] = best > a.psr_threshold ? 0 : 255;
+    if (a.psr_margin) a.psr_margin[tile_pixels + pixel] = static_cast<float>(best);
+  }
+}
+
+}  // namespace lsh

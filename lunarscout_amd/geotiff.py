@@ -1,0 +1,401 @@
+"""Minimal GeoTIFF reader and writer for stereographic lunar DEMs and products.
+
+The reference reads and writes rasters through rasterio/GDAL and interprets the
+CRS with pyproj (``src/lunarscout/geotiff.py:60-151``, ``products.py:97-177``);
+neither is available here, so this module reads the TIFF structure itself:
+
+* classic TIFF (``II*`` / ``MM*``), one band, strips or tiles;
+* compression none (1) or deflate (8, 32946); predictor none (1), horizontal
+  (2) or floating point (3, byte planes most significant first);
+* sample formats unsigned / signed integer and IEEE float, 8-64 bits;
+* georeferencing from ModelTiepoint + ModelPixelScale or ModelTransformation
+  (half-pixel shift for PixelIsPoint, as GDAL applies it) and the
+  stereographic GeoKeys GDAL writes (coordinate transformations 14, 15, 16:
+  origin latitude / longitude, scale, false easting / northing, semi-major
+  axis);
+* GDAL_NODATA.
+
+``write_geotiff`` writes what the reference writes for its products
+(``geotiff.py:190-199``): tiled 128x128, deflate, predictor 3 for floats and 2
+for integers, with the same stereographic GeoKeys.
+"""
+
+from __future__ import annotations
+
+import math
+import os
+import struct
+import tempfile
+import zlib
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from .errors import InputError, LunarscoutError
+from .geometry import DemGrid, ProjectionParameters
+
+# TIFF tags
+_WIDTH, _HEIGHT, _BITS, _COMPRESSION, _PHOTOMETRIC = 256, 257, 258, 259, 262
+_STRIP_OFFSETS, _SAMPLES, _ROWS_PER_STRIP, _STRIP_COUNTS = 273, 277, 278, 279
+_PLANAR, _PREDICTOR = 284, 317
+_TILE_W, _TILE_H, _TILE_OFFSETS, _TILE_COUNTS = 322, 323, 324, 325
+_SAMPLE_FORMAT = 339
+_PIXEL_SCALE, _TIEPOINT, _TRANSFORMATION = 33550, 33922, 34264
+_GEOKEYS, _GEO_DOUBLES, _GEO_ASCII, _GDAL_NODATA = 34735, 34736, 34737, 42113
+
+# GeoKeys
+_K_MODEL_TYPE, _K_RASTER_TYPE = 1024, 1025
+_K_GEOG_TYPE, _K_DATUM, _K_ANGULAR, _K_ELLIPSOID = 2048, 2050, 2054, 2056
+_K_SEMI_MAJOR, _K_SEMI_MINOR = 2057, 2058
+_K_PCS, _K_PROJECTION, _K_TRANSFORM, _K_LINEAR = 3072, 3074, 3075, 3076
+_K_STD_PARALLEL1, _K_NAT_LON, _K_NAT_LAT, _K_FALSE_E, _K_FALSE_N = 3078, 3080, 3081, 3082, 3083
+_K_FALSE_ORIGIN_E, _K_FALSE_ORIGIN_N = 3090, 3091
+_K_CENTER_LON, _K_CENTER_LAT, _K_SCALE_NAT, _K_SCALE_CENTER = 3088, 3089, 3092, 3093
+_K_POLE_LON = 3095
+_STEREOGRAPHIC_TRANSFORMS = {14, 15, 16}
+
+_TYPE_FORMATS = {1: "B", 2: "s", 3: "H", 4: "I", 5: "II", 6: "b", 8: "h", 9: "i", 10: "ii",
+                 11: "f", 12: "d", 16: "Q", 17: "q"}
+_TYPE_SIZES = {1: 1, 2: 1, 3: 2, 4: 4, 5: 8, 6: 1, 7: 1, 8: 2, 9: 4, 10: 8, 11: 4, 12: 8,
+               16: 8, 17: 8}
+
+
+class GeoTiffError(LunarscoutError, ValueError):
+    """A file that is not a GeoTIFF this reader understands."""
+
+
+@dataclass(frozen=True)
+class GeoTiffInfo:
+    """Georeferencing of one band: GDAL-order affine transform, projection, nodata."""
+
+    affine_transform: tuple[float, float, float, float, float, float] | None
+    projection: ProjectionParameters | None
+    nodata: float | None
+    width: int
+    height: int
+
+
+def _dtype(bits: int, sample_format: int) -> np.dtype:
+    kind = {1: "u", 2: "i", 3: "f"}.get(sample_format)
+    if kind is None or bits not in (8, 16, 32, 64) or (kind == "f" and bits < 32):
+        raise GeoTiffError("unsupported TIFF sample type", code="geotiff_data_type_unsupported",
+                           details={"bits": bits, "sample_format": sample_format})
+    return np.dtype(f"{kind}{bits // 8}")
+
+
+def _read_ifd(data: bytes, endian: str) -> dict[int, tuple]:
+    (offset,) = struct.unpack_from(endian + "I", data, 4)
+    (count,) = struct.unpack_from(endian + "H", data, offset)
+    tags: dict[int, tuple] = {}
+    for i in range(count):
+        tag, typ, n, value = struct.unpack_from(endian + "HHI4s", data, offset + 2 + 12 * i)
+        size = _TYPE_SIZES.get(typ)
+        if size is None:
+            continue
+        raw = value if size * n <= 4 else data[struct.unpack(endian + "I", value)[0]:][: size * n]
+        if typ == 2:
+            tags[tag] = (raw[:n].split(b"\0")[0].decode("ascii", "replace"),)
+        elif typ in (5, 10):
+            parts = struct.unpack(endian + _TYPE_FORMATS[typ][0] * (2 * n), raw[: size * n])
+            tags[tag] = tuple(parts[2 * k] / parts[2 * k + 1] if parts[2 * k + 1] else 0.0
+                              for k in range(n))
+        else:
+            tags[tag] = struct.unpack(endian + _TYPE_FORMATS[typ] * n, raw[: size * n])
+    return tags
+
+
+def _undo_predictor(block: np.ndarray, predictor: int, dtype: np.dtype, endian: str) -> np.ndarray:
+    """block: uint8 [rows, width * itemsize] as stored -> values [rows, width]."""
+    rows, row_bytes = block.shape
+    size = dtype.itemsize
+    width = row_bytes // size
+    if predictor == 3:
+        planes = np.cumsum(block, axis=1, dtype=np.uint8).reshape(rows, size, width)
+        ordered = planes.transpose(0, 2, 1)  # [rows, width, byte] most significant first
+        return np.ascontiguousarray(ordered).view(dtype.newbyteorder(">")).reshape(rows, width)
+    values = np.ascontiguousarray(block).view(dtype.newbyteorder(endian)).reshape(rows, width)
+    if predictor == 2:
+        values = np.cumsum(values, axis=1, dtype=values.dtype)
+    return values
+
+
+def read_geotiff(path) -> tuple[np.ndarray, GeoTiffInfo]:
+    """Band 1 of a GeoTIFF as its native dtype, with its georeferencing."""
+    path = Path(path).expanduser()
+    if not path.is_file():
+        raise GeoTiffError(f"GeoTIFF file does not exist: {path}", code="geotiff_file_not_found",
+                           details={"path": str(path)})
+    data = path.read_bytes()
+    if data[:4] == b"II*\0":
+        endian = "<"
+    elif data[:4] == b"MM\0*":
+        endian = ">"
+    else:
+        raise GeoTiffError(f"File is not a readable GeoTIFF: {path}",
+                           code="geotiff_unreadable_or_unsupported", details={"path": str(path)})
+    tags = _read_ifd(data, endian)
+    width, height = int(tags[_WIDTH][0]), int(tags[_HEIGHT][0])
+    samples = int(tags.get(_SAMPLES, (1,))[0])
+    planar = int(tags.get(_PLANAR, (1,))[0])
+    if samples != 1 and planar != 2:
+        raise GeoTiffError("only single-band or band-separate TIFFs are supported",
+                           code="geotiff_unreadable_or_unsupported", details={"path": str(path)})
+    dtype = _dtype(int(tags.get(_BITS, (8,))[0]), int(tags.get(_SAMPLE_FORMAT, (1,))[0]))
+    compression = int(tags.get(_COMPRESSION, (1,))[0])
+    if compression not in (1, 8, 32946):
+        raise GeoTiffError("unsupported TIFF compression", code="geotiff_unreadable_or_unsupported",
+                           details={"path": str(path), "compression": compression})
+    predictor = int(tags.get(_PREDICTOR, (1,))[0])
+    if _TILE_OFFSETS in tags:
+        bw, bh = int(tags[_TILE_W][0]), int(tags[_TILE_H][0])
+        offsets, counts = tags[_TILE_OFFSETS], tags[_TILE_COUNTS]
+    else:
+        bw, bh = width, int(tags.get(_ROWS_PER_STRIP, (height,))[0])
+        bh = min(bh, height)
+        offsets, counts = tags[_STRIP_OFFSETS], tags[_STRIP_COUNTS]
+    across = (width + bw - 1) // bw
+    down = (height + bh - 1) // bh
+    out = np.empty((height, width), dtype=dtype)
+    for index in range(across * down):  # band 1 comes first when band-separate
+        raw = data[offsets[index]: offsets[index] + counts[index]]
+        if compression != 1:
+            raw = zlib.decompress(raw)
+        by, bx = divmod(index, across)
+        rows = bh if _TILE_OFFSETS in tags else min(bh, height - by * bh)
+        block = np.frombuffer(raw, dtype=np.uint8)[: rows * bw * dtype.itemsize]
+        values = _undo_predictor(block.reshape(rows, bw * dtype.itemsize), predictor, dtype, endian)
+        y0, x0 = by * bh, bx * bw
+        h, w = min(rows, height - y0), min(bw, width - x0)
+        out[y0:y0 + h, x0:x0 + w] = values[:h, :w]
+    return out, GeoTiffInfo(_affine(tags), _projection(tags), _nodata(tags), width, height)
+
+
+def _affine(tags):
+    if _TRANSFORMATION in tags:
+        m = tags[_TRANSFORMATION]
+        return (m[3], m[0], m[1], m[7], m[4], m[5])
+    if _TIEPOINT in tags and _PIXEL_SCALE in tags:
+        i, j, _k, x, y, _z = tags[_TIEPOINT][:6]
+        sx, sy = tags[_PIXEL_SCALE][:2]
+        ox, oy = x - i * sx, y + j * sy
+        if _geokeys(tags).get(_K_RASTER_TYPE) == 2:  # PixelIsPoint: GDAL shifts by half a pixel
+            ox, oy = ox - 0.5 * sx, oy + 0.5 * sy
+        return (ox, sx, 0.0, oy, 0.0, -sy)
+    return None
+
+
+def _geokeys(tags) -> dict[int, object]:
+    if _GEOKEYS not in tags:
+        return {}
+    d = tags[_GEOKEYS]
+    doubles = tags.get(_GEO_DOUBLES, ())
+    ascii_ = tags.get(_GEO_ASCII, ("",))[0]
+    keys: dict[int, object] = {}
+    for k in range(int(d[3])):
+        key, location, count, value = d[4 + 4 * k: 8 + 4 * k]
+        if location == 0:
+            keys[key] = value
+        elif location == _GEO_DOUBLES:
+            keys[key] = doubles[value] if count == 1 else tuple(doubles[value:value + count])
+        elif location == _GEO_ASCII:
+            keys[key] = ascii_[value:value + count].rstrip("|")
+    return keys
+
+
+def _projection(tags) -> ProjectionParameters | None:
+    keys = _geokeys(tags)
+    if not keys:
+        return None
+    transform = keys.get(_K_TRANSFORM)
+    if transform not in _STEREOGRAPHIC_TRANSFORMS:
+        raise GeoTiffError("Horizon-derived products require a stereographic lunar DEM.",
+                           code="product_dem_projection_unsupported",
+                           details={"coordinate_transformation": transform})
+
+    def first(*names):
+        for n in names:
+            if n in keys:
+                return float(keys[n])
+        return None
+
+    lat0 = first(_K_NAT_LAT, _K_CENTER_LAT)
+    std = first(_K_STD_PARALLEL1)
+    if lat0 is None and std is not None and transform == 15:
+        lat0 = math.copysign(90.0, std)
+    lon0 = first(_K_POLE_LON, _K_NAT_LON, _K_CENTER_LON)
+    scale = first(_K_SCALE_NAT, _K_SCALE_CENTER)
+    if scale is None and std is not None and abs(abs(std) - 90.0) <= 1e-10:
+        scale = 1.0
+    fe = first(_K_FALSE_E, _K_FALSE_ORIGIN_E)
+    fn = first(_K_FALSE_N, _K_FALSE_ORIGIN_N)
+    radius = first(_K_SEMI_MAJOR)
+    values = (radius, lat0, lon0, scale, fe, fn)
+    if any(v is None or not math.isfinite(v) for v in values):
+        raise GeoTiffError("The DEM stereographic projection parameters are incomplete.",
+                           code="product_dem_projection_invalid", details={})
+    return ProjectionParameters(radius, math.radians(lat0), math.radians(lon0), scale, fe, fn)
+
+
+def _nodata(tags) -> float | None:
+    if _GDAL_NODATA not in tags:
+        return None
+    try:
+        return float(tags[_GDAL_NODATA][0].strip())
+    except ValueError:
+        return None
+
+
+def load_dem(path) -> DemGrid:
+    """A stereographic DEM as ``DemGrid`` (``products.py:97-177``)."""
+    dem_path = Path(path).expanduser().resolve()
+    if not dem_path.is_file():
+        raise InputError("The DEM path does not identify a file.", code="product_dem_not_found",
+                         details={"path": str(dem_path)})
+    values, info = read_geotiff(dem_path)
+    if info.affine_transform is None or info.projection is None:
+        raise GeoTiffError("The DEM must have complete geospatial metadata.",
+                           code="product_dem_not_georeferenced", details={"path": str(dem_path)})
+    return DemGrid(np.ascontiguousarray(values, dtype=np.float32),
+                   np.asarray(info.affine_transform, dtype=np.float64), info.projection)
+
+
+# -- writer -----------------------------------------------------------------------------------
+
+def _entry(tag, typ, values, endian, blobs, base):
+    fmt = _TYPE_FORMATS[typ]
+    if typ == 2:
+        raw = values.encode("ascii") + b"\0"
+        n = len(raw)
+    else:
+        n = len(values)
+        raw = struct.pack(endian + fmt * n, *values)
+    if len(raw) <= 4:
+        return struct.pack(endian + "HHI", tag, typ, n) + raw.ljust(4, b"\0")
+    offset = base + sum(len(b) for b in blobs)
+    blobs.append(raw + (b"\0" if len(raw) % 2 else b""))
+    return struct.pack(endian + "HHII", tag, typ, n, offset)
+
+
+def _encode_block(values: np.ndarray, predictor: int) -> bytes:
+    rows, width = values.shape
+    if predictor == 3:
+        be = np.ascontiguousarray(values.astype(values.dtype.newbyteorder(">")))
+        planes = be.view(np.uint8).reshape(rows, width, values.dtype.itemsize).transpose(0, 2, 1)
+        planes = np.ascontiguousarray(planes).reshape(rows, -1)
+        diff = np.diff(planes, axis=1, prepend=np.zeros((rows, 1), np.uint8)).astype(np.uint8)
+        return zlib.compress(diff.tobytes(), 6)
+    le = np.ascontiguousarray(values.astype(values.dtype.newbyteorder("<")))
+    diff = np.diff(le, axis=1, prepend=np.zeros((rows, 1), le.dtype)).astype(le.dtype)
+    return zlib.compress(diff.tobytes(), 6)
+
+
+def write_geotiff(path, array: np.ndarray, affine_transform, projection: ProjectionParameters, *,
+                  nodata: float | None = None, overwrite: bool = False) -> Path:
+    """Atomically write one band as a tiled 128x128 deflate GeoTIFF (geotiff.py:202-260)."""
+    path = Path(path).expanduser().resolve()
+    values = np.asarray(array)
+    if values.ndim != 2:
+        raise InputError("GeoTIFF output must be a two-dimensional array.",
+                         code="geotiff_invalid_array_shape", details={"shape": list(values.shape)})
+    if values.dtype == np.bool_ or values.dtype.kind not in "uif":
+        raise InputError("unsupported GeoTIFF data type", code="geotiff_data_type_unsupported",
+                         details={"dtype": str(values.dtype)})
+    if path.exists() and not overwrite:
+        raise InputError(f"GeoTIFF output already exists: {path}", code="output_exists",
+                         details={"path": str(path)})
+    endian = "<"
+    height, width = values.shape
+    tile = 128
+    predictor = 3 if values.dtype.kind == "f" else 2
+    across, down = (width + tile - 1) // tile, (height + tile - 1) // tile
+    blocks = []
+    for by in range(down):
+        for bx in range(across):
+            block = np.zeros((tile, tile), dtype=values.dtype)
+            part = values[by * tile:(by + 1) * tile, bx * tile:(bx + 1) * tile]
+            block[: part.shape[0], : part.shape[1]] = part
+            blocks.append(_encode_block(block, predictor))
+    t = list(affine_transform)
+    geokeys, doubles = _stereographic_geokeys(projection)
+    fmt_code = {"u": 1, "i": 2, "f": 3}[values.dtype.kind]
+    tags = [
+        (_WIDTH, 4, (width,)), (_HEIGHT, 4, (height,)), (_BITS, 3, (values.dtype.itemsize * 8,)),
+        (_COMPRESSION, 3, (8,)), (_PHOTOMETRIC, 3, (1,)), (_SAMPLES, 3, (1,)), (_PLANAR, 3, (1,)),
+        (_PREDICTOR, 3, (predictor,)), (_TILE_W, 3, (tile,)), (_TILE_H, 3, (tile,)),
+        (_TILE_OFFSETS, 4, None), (_TILE_COUNTS, 4, tuple(len(b) for b in blocks)),
+        (_SAMPLE_FORMAT, 3, (fmt_code,)),
+        (_PIXEL_SCALE, 12, (t[1], -t[5], 0.0)), (_TIEPOINT, 12, (0.0, 0.0, 0.0, t[0], t[3], 0.0)),
+        (_GEOKEYS, 3, geokeys), (_GEO_DOUBLES, 12, doubles),
+    ]
+    if t[2] != 0.0 or t[4] != 0.0:  # rotated or sheared: a full model transformation
+        tags = [x for x in tags if x[0] not in (_PIXEL_SCALE, _TIEPOINT)]
+        tags.append((_TRANSFORMATION, 12, (t[1], t[2], 0.0, t[0], t[4], t[5], 0.0, t[3],
+                                           0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0)))
+    if nodata is not None:
+        tags.append((_GDAL_NODATA, 2, repr(float(nodata))))
+    tags.sort(key=lambda x: x[0])
+    header = 8
+    data_start = header + 2 + 12 * len(tags) + 4
+    # out-of-line tag values follow the IFD, then the compressed blocks; the
+    # values' sizes do not depend on the offsets, so one sizing pass places them
+    sizing: list[bytes] = []
+    for tag, typ, vals in tags:
+        _entry(tag, typ, tuple(0 for _ in blocks) if vals is None else vals, endian, sizing,
+               data_start)
+    position = data_start + sum(len(b) for b in sizing)
+    positions = []
+    for b in blocks:
+        positions.append(position)
+        position += len(b)
+    blobs: list[bytes] = []
+    entries = [_entry(tag, typ, tuple(positions) if vals is None else vals, endian, blobs, data_start)
+               for tag, typ, vals in tags]
+    content = (b"II*\0" + struct.pack("<I", header) + struct.pack("<H", len(tags))
+               + b"".join(entries) + struct.pack("<I", 0) + b"".join(blobs) + b"".join(blocks))
+    path.parent.mkdir(parents=True, exist_ok=True)
+    fd, tmp = tempfile.mkstemp(prefix=f".{path.name}.", suffix=".tmp.tif", dir=path.parent)
+    try:
+        with os.fdopen(fd, "wb") as handle:
+            handle.write(content)
+            handle.flush()
+            os.fsync(handle.fileno())
+        os.replace(tmp, path)
+    except BaseException:
+        try:
+            os.unlink(tmp)
+        except OSError:
+            pass
+        raise
+    return path
+
+
+def _stereographic_geokeys(p: ProjectionParameters):
+    lat0 = math.degrees(p.latitude_origin_rad)
+    polar = abs(abs(lat0) - 90.0) <= 1e-10
+    doubles = [lat0, math.degrees(p.longitude_origin_rad), p.scale, p.false_easting_m,
+               p.false_northing_m, p.radius_m, p.radius_m]
+    lon_key = _K_POLE_LON if polar else _K_NAT_LON
+    keys = [
+        (_K_MODEL_TYPE, 0, 1, 1), (_K_RASTER_TYPE, 0, 1, 1),
+        (_K_GEOG_TYPE, 0, 1, 32767), (_K_DATUM, 0, 1, 32767), (_K_ANGULAR, 0, 1, 9102),
+        (_K_ELLIPSOID, 0, 1, 32767), (_K_SEMI_MAJOR, _GEO_DOUBLES, 1, 5),
+        (_K_SEMI_MINOR, _GEO_DOUBLES, 1, 6),
+        (_K_PCS, 0, 1, 32767), (_K_PROJECTION, 0, 1, 32767),
+        (_K_TRANSFORM, 0, 1, 15 if polar else 14), (_K_LINEAR, 0, 1, 9001),
+        (_K_NAT_LAT, _GEO_DOUBLES, 1, 0), (lon_key, _GEO_DOUBLES, 1, 1),
+        (_K_SCALE_NAT, _GEO_DOUBLES, 1, 2), (_K_FALSE_E, _GEO_DOUBLES, 1, 3),
+        (_K_FALSE_N, _GEO_DOUBLES, 1, 4),
+    ]
+    keys.sort(key=lambda k: k[0])
+    flat = [1, 1, 0, len(keys)]
+    for k in keys:
+        flat.extend(k)
+    return tuple(flat), tuple(doubles)
+
+
+def write_dem(path, dem: DemGrid, *, overwrite: bool = False) -> Path:
+    """A ``DemGrid`` as a float32 GeoTIFF that ``load_dem`` reads back exactly."""
+    return write_geotiff(path, np.asarray(dem.elevation_m, dtype=np.float32),
+                         tuple(float(v) for v in dem.geo_transform), dem.projection,
+                         overwrite=overwrite)

@@ -73,6 +73,16 @@ class HipSession(_lib.Session):
         self._output: DeviceBuffer | None = None
         self.last_production_timings: dict | None = None
 
+    def close(self) -> None:
+        """Free the session's buffers, its resident DEMs and the stream."""
+        for name in ("_segments", "_output"):
+            buf = getattr(self, name, None)
+            if buf is not None:
+                buf.free()
+                setattr(self, name, None)
+        self._dems = {}
+        super().close()
+
     # -- DEM residency ---------------------------------------------------
     def dem_id(self, dem: DemGrid, mips: np.ndarray | None = None) -> int:
         """Register ``dem`` on the device once (pyramid built there unless given)."""

@@ -245,3 +245,138 @@ class HipLightingSession:
         finally:
             out.free()
         return np.ascontiguousarray(mask[:valid_height, :valid_width])
+
+
+class CpuLightingSession:
+    """The same lighting on host threads: the ``"cpu"`` backend.
+
+    Drop-in for the reference's ``LightmapCpuSession`` (lightmap_cpu.py:193-237)
+    and its CPU PSR path (``compute_psr_patch_reference``, psr.py:94-140). The
+    arithmetic is the HIP kernel's (``lsh_lighting_core.h``) compiled for the
+    host, so the two backends agree up to the host and device math libraries.
+    It never touches a GPU.
+    """
+
+    def __init__(self, *, threads: int = 0, time_batch_size: int = 64,
+                 sun_half_angle_deg: float = SUN_HALF_ANGLE_DEG,
+                 sun_angular_size_deg: float = SUN_ANGULAR_SIZE_DEG):
+        if time_batch_size < 1:
+            raise ValueError("time_batch_size must be positive")
+        self.lib = _lib.load_library()
+        self.threads = int(threads)
+        self.time_batch_size = int(time_batch_size)
+        self.sun_half_angle_deg = float(sun_half_angle_deg)
+        self.sun_angular_size_deg = float(sun_angular_size_deg)
+
+    def lighting_tiles(self, dem: DemGrid, tiles, horizons: np.ndarray, *, sun=None, earth=None,
+                       psr_vectors=None, azimuth_major: bool = False, sun_bytes: bool = False,
+                       sun_fraction: bool = False, sun_mean: bool = True,
+                       earth_margin: bool = False, earth_visible: bool = True, psr: bool = True,
+                       psr_margin: bool = False) -> dict[str, np.ndarray]:
+        """Host arrays in, host arrays out; shapes as ``HipLightingSession.lighting_tiles``."""
+        tile_arr = _tile_array(tiles)
+        n = len(tile_arr)
+        hz = np.ascontiguousarray(horizons, dtype=np.float32)
+        N = hz.shape[-2] if azimuth_major else hz.shape[-1]
+        if hz.size != n * PATCH_PIXELS * N:
+            raise ValueError("horizons must hold [tile][16384][N] (or [tile][N][16384]) values")
+        vecs = {k: (validate_vectors(v) if v is not None else None)
+                for k, v in (("sun", sun), ("earth", earth), ("psr", psr_vectors))}
+        T = 0 if vecs["sun"] is None else len(vecs["sun"])
+        E = 0 if vecs["earth"] is None else len(vecs["earth"])
+        P = 0 if vecs["psr"] is None else len(vecs["psr"])
+        want = {
+            "sun_bytes": (sun_bytes and T, (n, T, PATCH_PIXELS), np.uint8, _lib.LIGHT_SUN_BYTES),
+            "sun_fraction": (sun_fraction and T, (n, T, PATCH_PIXELS), np.float32, _lib.LIGHT_SUN_FRACTION),
+            "sun_mean": (sun_mean and T, (n, PATCH_PIXELS), np.float32, _lib.LIGHT_SUN_MEAN),
+            "earth_margin": (earth_margin and E, (n, E, PATCH_PIXELS), np.float32, _lib.LIGHT_EARTH_MARGIN),
+            "earth_visible": (earth_visible and E, (n, PATCH_PIXELS), np.float32, _lib.LIGHT_EARTH_VISIBLE),
+            "psr": (psr and P, (n, PATCH_PIXELS), np.uint8, _lib.LIGHT_PSR),
+            "psr_margin": (psr_margin and P, (n, PATCH_PIXELS), np.float32, _lib.LIGHT_PSR_MARGIN),
+        }
+        out, ptrs, flags = {}, {}, 0
+        for name, (enabled, shape, dtype, flag) in want.items():
+            if enabled:
+                out[name] = np.empty(shape, dtype)
+                ptrs[name] = out[name].ctypes.data
+                flags |= flag
+            else:
+                ptrs[name] = None
+        elevation = np.ascontiguousarray(dem.elevation_m, dtype=np.float32)
+        gt = np.ascontiguousarray(dem.geo_transform, dtype=np.float64)
+        proj = dem.projection.as_array()
+
+        def ptr(v):
+            return None if v is None else v.ctypes.data
+
+        check(self.lib.lsh_lighting_tiles_host(
+            elevation.ctypes.data, dem.width, dem.height, gt.ctypes.data, proj.ctypes.data,
+            tile_arr.ctypes.data, n, hz.ctypes.data, N, 1 if azimuth_major else 0,
+            ptr(vecs["sun"]), T, ptr(vecs["earth"]), E, ptr(vecs["psr"]), P,
+            self.sun_half_angle_deg, self.sun_angular_size_deg, flags, self.threads,
+            ptrs["sun_bytes"], ptrs["sun_fraction"], ptrs["sun_mean"], ptrs["earth_margin"],
+            ptrs["earth_visible"], ptrs["psr"], ptrs["psr_margin"]), "lsh_lighting_tiles_host")
+        return out
+
+    def _signal_batches(self, dem, horizons_deg, vectors_m, *, tile_y, tile_x, valid_height,
+                        valid_width, margin: bool):
+        HipLightingSession._check_patch(dem, tile_y, tile_x, valid_height, valid_width)
+        vectors = validate_vectors(vectors_m)
+        hz = _finite_patch(horizons_deg)
+        for start in range(0, len(vectors), self.time_batch_size):
+            batch = vectors[start:start + self.time_batch_size]
+            if margin:
+                values = self.lighting_tiles(dem, [(tile_x, tile_y)], hz, earth=batch,
+                                             sun_mean=False, earth_visible=False, psr=False,
+                                             earth_margin=True)["earth_margin"]
+            else:
+                values = self.lighting_tiles(dem, [(tile_x, tile_y)], hz, sun=batch,
+                                             sun_mean=False, earth_visible=False, psr=False,
+                                             sun_fraction=True)["sun_fraction"]
+            yield values.reshape(len(batch), PATCH_SIZE, PATCH_SIZE)[:, :valid_height, :valid_width]
+
+    iter_patch_fraction_tiles = HipLightingSession.iter_patch_fraction_tiles
+    iter_patch_margin_tiles = HipLightingSession.iter_patch_margin_tiles
+    iter_patch_tiles = HipLightingSession.iter_patch_tiles
+
+    def compute_patch(self, dem, horizons_deg, reduced_sun_vectors_m, *, tile_y, tile_x,
+                      valid_height=PATCH_SIZE, valid_width=PATCH_SIZE) -> np.ndarray:
+        """uint8 PSR mask [valid_height, valid_width] (255 = PSR)."""
+        HipLightingSession._check_patch(dem, tile_y, tile_x, valid_height, valid_width)
+        vectors = validate_vectors(reduced_sun_vectors_m)
+        mask = self.lighting_tiles(dem, [(tile_x, tile_y)], _finite_patch(horizons_deg),
+                                   psr_vectors=vectors, sun_mean=False, earth_visible=False,
+                                   psr=True)["psr"].reshape(PATCH_SIZE, PATCH_SIZE)
+        return np.ascontiguousarray(mask[:valid_height, :valid_width])
+
+
+def _finite_patch(horizons_deg) -> np.ndarray:
+    horizons = np.ascontiguousarray(horizons_deg, dtype=np.float32)
+    if horizons.ndim != 3 or horizons.shape[:2] != (PATCH_SIZE, PATCH_SIZE):
+        raise ValueError("horizons must have shape (128, 128, N)")
+    if not np.all(np.isfinite(horizons)):
+        raise ValueError("horizons must contain finite elevations")
+    return horizons.reshape(1, PATCH_PIXELS, horizons.shape[2])
+
+
+def select_lighting_session(backend: str = "auto", **options):
+    """The lighting calculator for a backend (lightmap_pipeline.py:81-108, psr_pipeline.py:218-258).
+
+    ``"cpu"`` never probes the GPU; ``"hip"`` never falls back (a missing
+    library or device raises ``HipBackendError``); ``"auto"`` uses the GPU when a
+    session can be created and the host path otherwise. Returns
+    ``(session, selected_backend)``.
+    """
+    from .errors import HipBackendError
+
+    if backend not in ("auto", "cpu", "hip"):
+        raise ValueError("backend must be 'auto', 'cpu', or 'hip'")
+    host_options = {k: v for k, v in options.items() if k != "device_id"}
+    if backend == "cpu":
+        return CpuLightingSession(**host_options), "cpu"
+    try:
+        return HipLightingSession(**options), "hip"
+    except HipBackendError:
+        if backend == "hip":
+            raise
+        return CpuLightingSession(**host_options), "cpu"

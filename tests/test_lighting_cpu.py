@@ -1,0 +1,106 @@
+"""The "cpu" lighting backend (host build of the lighting kernel) and backend selection.
+
+Bars as for the GPU path: C# BuilderSunFraction bytes and C# PSR tiles
+byte-exact; sun fraction within 1e-3 and Earth margin within 1e-4 of the
+float64 oracle; PSR identical except near the threshold.
+"""
+
+from __future__ import annotations
+
+import base64
+
+import numpy as np
+import pytest
+
+import oracle
+from tests import fixtures as fx
+from lunarscout_amd import synthetic
+from lunarscout_amd.geometry import DemGrid, ProjectionParameters, local_direction_vector
+from lunarscout_amd.lighting import CpuLightingSession, select_lighting_session
+
+
+def _unit_dem():
+    return DemGrid(np.zeros((1, 1), np.float32),
+                   np.asarray((1000.0, 1.0, 0.0, -1000.0, 0.0, -1.0), np.float64),
+                   ProjectionParameters.south_polar())
+
+
+def test_sun_fraction_bytes_match_csharp_builder():
+    dem, cpu = _unit_dem(), CpuLightingSession(threads=2)
+    for case in fx.lightmap_csharp()["cases"]:
+        horizon = fx.decode_f32(case["horizons_float32_base64"])
+        tile = np.broadcast_to(horizon, (128, 128, horizon.size))
+        vectors = np.stack([local_direction_vector(dem, 0, 0, s["azimuth_deg"], s["elevation_deg"],
+                                                   1.5e11) for s in case["samples"]])
+        got = np.stack(list(cpu.iter_patch_tiles(dem, tile, vectors, tile_y=0, tile_x=0,
+                                                 valid_height=1, valid_width=1)))[:, 0, 0]
+        np.testing.assert_array_equal(got, [s["encoded_byte"] for s in case["samples"]],
+                                      err_msg=case["name"])
+
+
+def test_psr_matches_csharp_kernel_fixture():
+    art = fx.psr_csharp()
+    p = art["projection"]
+    x = np.arange(128, dtype=np.float64)[None, :]
+    y = np.arange(128, dtype=np.float64)[:, None]
+    dem = DemGrid(np.ascontiguousarray((x - y) * 0.1, dtype=np.float32),
+                  np.asarray(art["geotransform"], np.float64),
+                  ProjectionParameters(p["radius_m"], p["latitude_origin_rad"],
+                                       p["longitude_origin_rad"], p["scale"],
+                                       p["false_easting_m"], p["false_northing_m"]))
+    pixel = np.arange(128 * 128, dtype=np.int32)[:, None]
+    azimuth = np.arange(1440, dtype=np.int32)[None, :]
+    cpu = CpuLightingSession()
+    for case in art["cases"]:
+        if case["name"] == "constant_shadow":
+            hz = np.full((128 * 128, 1440), 0.3, np.float32)
+        else:
+            hz = (np.float32(0.45) + np.float32(0.01) * (pixel % 23)
+                  + np.float32(0.0001) * (azimuth % 17)).astype(np.float32)
+            if case["name"] == "compressed_quantized_mixed":
+                scaled = hz * np.float32(32767.0 / 50.0)
+                hz = (np.floor(scaled + np.float32(0.5)).astype(np.int16)
+                      * np.float32(50.0 / 32767.0)).astype(np.float32)
+        expected = np.frombuffer(base64.b64decode(case["output_base64"]), dtype=np.uint8).reshape(128, 128)
+        got = cpu.compute_patch(dem, hz.reshape(128, 128, 1440),
+                                np.asarray(case["sun_vectors_m"], np.float64).reshape(-1, 3),
+                                tile_y=0, tile_x=0)
+        np.testing.assert_array_equal(got, expected, err_msg=case["name"])
+
+
+def test_streamed_lighting_matches_the_float64_oracle():
+    dem = synthetic.fbm_crater_dem(256, 5.0, 1)
+    rng = np.random.default_rng(0)
+    az = np.radians(np.arange(1440) * 0.25)
+    hz = (rng.uniform(-1.5, 5.5, (128, 128, 1)) + 1.5 * np.sin(3 * az)[None, None, :]).astype(np.float32)
+    sun = synthetic.sun_vectors(dem, 48, 6.0, seed=3)
+    earth = synthetic.earth_vectors(48, 6.0, seed=4)
+    reduced, _ = oracle.psr_reduce(dem, sun)
+    out = CpuLightingSession(threads=4).lighting_tiles(
+        dem, [(0, 128)], hz.reshape(1, 16384, 1440), sun=sun, earth=earth, psr_vectors=reduced,
+        sun_fraction=True, earth_margin=True, earth_visible=True, psr=True)
+    ref = oracle.lighting_patch(dem, hz, sun, tile_y=128, tile_x=0)
+    assert np.abs(out["sun_fraction"][0].reshape(-1, 128, 128) - ref["fraction"]).max() <= 1e-3
+    eref = oracle.lighting_patch(dem, hz, earth, tile_y=128, tile_x=0)
+    assert np.abs(out["earth_margin"][0].reshape(-1, 128, 128) - eref["margin"]).max() <= 1e-4
+    mask, max_margin = oracle.psr_patch(dem, hz, reduced, tile_y=128, tile_x=0)
+    differ = out["psr"][0].reshape(128, 128) != mask
+    near = np.abs(max_margin - (-float(np.float32(0.545)) / 2)) <= 1e-3
+    assert not np.any(differ & ~near)
+    assert 0 < int((mask == 255).sum()) < 128 * 128  # the case exercises both classes
+
+
+def test_backend_selection():
+    session, selected = select_lighting_session("cpu")
+    assert selected == "cpu" and isinstance(session, CpuLightingSession)
+    with pytest.raises(ValueError):
+        select_lighting_session("cuda")
+    from lunarscout_amd import _lib
+
+    if _lib.device_count() == 0:
+        _, selected = select_lighting_session("auto")
+        assert selected == "cpu"
+        from lunarscout_amd.errors import HipBackendError
+
+        with pytest.raises(HipBackendError):
+            select_lighting_session("hip")
