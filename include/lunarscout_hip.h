@@ -148,6 +148,54 @@ int lsh_horizon_tile_full(lsh_session* session, const int32_t* dem_ids, int32_t 
                           double maximum_distance_m, float observer_elevation_m,
                           int32_t flags, float* output, int64_t* counters);
 
+/* Batched horizon generation: many tiles of many sites per launch.
+ * It stands in for the reference's per-tile loop (generate_patch_horizons over
+ * the bounded pipeline, generator.py:11-36, pipeline.py:127-450) when many
+ * landing-site patches are generated together; the per-ray method and results
+ * are those of lsh_horizon_tile, bit for bit.
+ * A batch owns per-tile segment slots, the device site and tile tables, a
+ * pinned staging copy of the tile list, the work-queue counters and a hipGraph
+ * of the per-batch sequence (tile-list upload, FP64 segments of every tile, a
+ * counter reset, the persistent march). */
+typedef struct lsh_batch lsh_batch;
+
+typedef struct {
+  int32_t dem_ids[4];         /* the site's ordered DEM list, primary first */
+  int32_t dem_count;          /* 1..4 */
+  float observer_elevation_m; /* observer height above the primary DEM */
+} lsh_batch_site;
+
+typedef struct {
+  int32_t site;        /* index into the batch's site list */
+  int32_t tile_column; /* 128x128 tile origin in the site's primary DEM */
+  int32_t tile_row;
+  int32_t reserved;    /* must be 0 */
+} lsh_batch_tile;
+
+/* Kernel shapes the autotuner may choose for lsh_batch_run's `config`:
+ * block_x x block_y pixels per wavefront (64 rays) and `azimuths` wavefronts
+ * (consecutive azimuths) per workgroup. Shape 0 is the plain kernel's. */
+int lsh_batch_shape_count(int32_t* count);
+int lsh_batch_shape(int32_t config, int32_t* block_x, int32_t* block_y, int32_t* azimuths);
+
+int lsh_batch_create(lsh_session* session, const lsh_batch_site* sites, int32_t site_count,
+                     int32_t max_tiles, int32_t azimuth_count, double maximum_distance_m,
+                     lsh_batch** batch);
+
+/* Horizons of tile_count <= max_tiles tiles into output (device float32
+ * [tile_count][16384][azimuth_count], pixel-major per tile), in one persistent
+ * launch: `workgroups` resident workgroups (a multiple of 8; <= 0 picks one
+ * per the kernel's occupancy) drain per-XCD work queues of (azimuth group,
+ * tile, pixel block) items, an XCD first taking the items of its own angular
+ * sector and then stealing. flags: 0 or LSH_HORIZON_DEGREES. use_graph = 1
+ * replays the captured sequence (captured again when tile_count, config,
+ * workgroups, flags or output change). Stream ordered; the call returns after
+ * enqueueing (lsh_synchronize waits). */
+int lsh_batch_run(lsh_batch* batch, const lsh_batch_tile* tiles, int32_t tile_count,
+                  int32_t config, int32_t workgroups, int32_t flags, int32_t use_graph,
+                  float* output);
+int lsh_batch_destroy(lsh_batch* batch);
+
 /* Lighting of stored horizon tiles in one streamed pass (config of the
  * reference's LightmapCudaSession.iter_patch_tiles / iter_patch_fraction_tiles /
  * iter_patch_margin_tiles, lightmap_cuda.py:187-276, and

@@ -80,6 +80,11 @@ SIGNATURES = {
     "lsh_cbin_encode": [_vp, _i32, _i32, _vp, _i64, ctypes.POINTER(_i64), _i32],
     "lsh_cbin_decode": [_vp, _i64, _i32, _i32, _vp, _i32],
     "lsh_cbin_complete": [_vp, _i64, _i32, _i32, ctypes.POINTER(_i32)],
+    "lsh_batch_shape_count": [ctypes.POINTER(_i32)],
+    "lsh_batch_shape": [_i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
+    "lsh_batch_create": [_vp, _vp, _i32, _i32, _i32, _f64, ctypes.POINTER(_vp)],
+    "lsh_batch_run": [_vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp],
+    "lsh_batch_destroy": [_vp],
     "lsh_probe_copy": [_vp, _vp, _vp, ctypes.c_size_t],
     "lsh_probe_gather": [_vp, _vp, _i32, _i32, _i32, _i32, _vp],
     "lsh_event_create": [_vp, ctypes.POINTER(_vp)],

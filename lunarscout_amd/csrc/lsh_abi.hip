@@ -24,23 +24,6 @@ using lsh::Dem;
 using lsh::Session;
 using lsh::set_error;
 
-#define LSH_CHECK_HIP(expr, what)                                              \
-  do {                                                                         \
-    hipError_t _e = (expr);                                                    \
-    if (_e != hipSuccess) {                                                    \
-      set_error(std::string(what) + ": " + hipGetErrorString(_e));             \
-      return _e == hipErrorOutOfMemory ? LSH_ENOMEM : LSH_EDEVICE;             \
-    }                                                                          \
-  } while (0)
-
-#define LSH_REQUIRE(cond, message) \
-  do {                             \
-    if (!(cond)) {                 \
-      set_error(message);          \
-      return LSH_EINVAL;           \
-    }                              \
-  } while (0)
-
 namespace {
 
 int pyramid_layout(int w, int h, int32_t (*levels)[4], int cap, int64_t* mip_count) {
@@ -63,6 +46,10 @@ int pyramid_layout(int w, int h, int32_t (*levels)[4], int cap, int64_t* mip_cou
   return count;
 }
 
+}  // namespace
+
+namespace lsh {
+
 Dem* find_dem(Session* s, int32_t id) {
   if (!s || id < 0 || id >= static_cast<int32_t>(s->dems.size()) || !s->dems[id].live) return nullptr;
   return &s->dems[id];
@@ -73,7 +60,10 @@ int select_device(Session* s) {
   return LSH_OK;
 }
 
-}  // namespace
+}  // namespace lsh
+
+using lsh::find_dem;
+using lsh::select_device;
 
 namespace lsh {
 
@@ -102,6 +92,20 @@ int upload_tiles(Session* s, const lsh_tile* tiles, int32_t count, const lsh_til
                 "copy tile list");
   *device_tiles = s->tile_scratch;
   return LSH_OK;
+}
+
+SegmentDem segment_dem(const Dem& d, double maximum_distance_m) {
+  SegmentDem g{};
+  g.elevation = d.level0;
+  g.width = d.width;
+  g.height = d.height;
+  std::memcpy(g.t, d.geo_transform, sizeof(g.t));
+  std::memcpy(g.p, d.projection, sizeof(g.p));
+  // DemGrid.map_resolution_m and the ray limit (geometry.py:78-83, 452-462)
+  g.map_resolution = (std::hypot(g.t[1], g.t[4]) + std::hypot(g.t[2], g.t[5])) * 0.5;
+  double size_m = std::fmin(d.width * g.map_resolution, d.height * g.map_resolution);
+  g.ray_limit = std::fmin(maximum_distance_m, size_m * 1.2);
+  return g;
 }
 
 }  // namespace lsh
@@ -407,16 +411,7 @@ int lsh_segments_tile(lsh_session* session, const int32_t* dem_ids, int32_t dem_
   for (int i = 0; i < dem_count; ++i) {
     Dem* d = find_dem(s, dem_ids[i]);
     LSH_REQUIRE(d, "unknown DEM id");
-    lsh::SegmentDem& g = a.dem[i];
-    g.elevation = d->level0;
-    g.width = d->width;
-    g.height = d->height;
-    std::memcpy(g.t, d->geo_transform, sizeof(g.t));
-    std::memcpy(g.p, d->projection, sizeof(g.p));
-    // DemGrid.map_resolution_m and the ray limit (geometry.py:78-83, 452-462)
-    g.map_resolution = (std::hypot(g.t[1], g.t[4]) + std::hypot(g.t[2], g.t[5])) * 0.5;
-    double size_m = std::fmin(d->width * g.map_resolution, d->height * g.map_resolution);
-    g.ray_limit = std::fmin(maximum_distance_m, size_m * 1.2);
+    a.dem[i] = lsh::segment_dem(*d, maximum_distance_m);
   }
   if (int r = select_device(s)) return r;
   LSH_CHECK_HIP(lsh::launch_segments(a, s->stream), "segment kernel launch");

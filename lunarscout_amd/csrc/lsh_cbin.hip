@@ -108,20 +108,12 @@ void parallel_chunks(int threads, int64_t items, F&& body) {
 
 using namespace lsh;
 
-#define LSH_REQUIRE_C(cond, message) \
-  do {                               \
-    if (!(cond)) {                   \
-      set_error(message);            \
-      return LSH_EINVAL;             \
-    }                                \
-  } while (0)
-
 extern "C" {
 
 int lsh_cbin_encode(const float* degrees, int32_t pixels, int32_t azimuths, uint8_t* payload,
                     int64_t capacity, int64_t* payload_bytes, int32_t threads) {
-  LSH_REQUIRE_C(degrees && payload && payload_bytes, "invalid arguments");
-  LSH_REQUIRE_C(pixels > 0 && azimuths > 0 && azimuths <= 32767, "invalid tile shape");
+  LSH_REQUIRE(degrees && payload && payload_bytes, "invalid arguments");
+  LSH_REQUIRE(pixels > 0 && azimuths > 0 && azimuths <= 32767, "invalid tile shape");
   const int64_t block_max = 2 * static_cast<int64_t>(azimuths);
   const int nt = thread_count(threads, pixels);
   std::vector<std::vector<uint8_t>> parts(nt);
@@ -144,10 +136,10 @@ int lsh_cbin_encode(const float* degrees, int32_t pixels, int32_t azimuths, uint
   });
   int64_t total = 0;
   for (int t = 0; t < nt; ++t) {
-    LSH_REQUIRE_C(!failed[t], "degrees must not contain NaN values");
+    LSH_REQUIRE(!failed[t], "degrees must not contain NaN values");
     total += static_cast<int64_t>(parts[t].size());
   }
-  LSH_REQUIRE_C(total <= capacity, "payload capacity is too small");
+  LSH_REQUIRE(total <= capacity, "payload capacity is too small");
   int64_t offset = 0;
   for (int t = 0; t < nt; ++t) {
     std::memcpy(payload + offset, parts[t].data(), parts[t].size());
@@ -159,19 +151,19 @@ int lsh_cbin_encode(const float* degrees, int32_t pixels, int32_t azimuths, uint
 
 int lsh_cbin_decode(const uint8_t* payload, int64_t bytes, int32_t pixels, int32_t azimuths,
                     float* degrees, int32_t threads) {
-  LSH_REQUIRE_C(payload && degrees && bytes >= 0, "invalid arguments");
-  LSH_REQUIRE_C(pixels > 0 && azimuths > 0 && azimuths <= 32767, "invalid tile shape");
+  LSH_REQUIRE(payload && degrees && bytes >= 0, "invalid arguments");
+  LSH_REQUIRE(pixels > 0 && azimuths > 0 && azimuths <= 32767, "invalid tile shape");
   std::vector<int64_t> offsets(static_cast<size_t>(pixels));
   int64_t off = 0;
   for (int32_t p = 0; p < pixels; ++p) {
-    LSH_REQUIRE_C(off + 2 <= bytes, "compressed horizon ended while reading a block length");
+    LSH_REQUIRE(off + 2 <= bytes, "compressed horizon ended while reading a block length");
     const int length = payload[off] | (payload[off + 1] << 8);
-    LSH_REQUIRE_C(length >= 2 && length <= 2 * azimuths, "compressed horizon block length is invalid");
+    LSH_REQUIRE(length >= 2 && length <= 2 * azimuths, "compressed horizon block length is invalid");
     offsets[p] = off + 2;
     off += 2 + length;
-    LSH_REQUIRE_C(off <= bytes, "compressed horizon ended inside a block");
+    LSH_REQUIRE(off <= bytes, "compressed horizon ended inside a block");
   }
-  LSH_REQUIRE_C(off == bytes, "compressed horizon contains trailing bytes");
+  LSH_REQUIRE(off == bytes, "compressed horizon contains trailing bytes");
   const int nt = thread_count(threads, pixels);
   std::vector<int> failed(nt, 0);
   parallel_chunks(nt, pixels, [&](int t, int64_t begin, int64_t end) {
@@ -182,13 +174,13 @@ int lsh_cbin_decode(const uint8_t* payload, int64_t bytes, int32_t pixels, int32
     }
   });
   for (int t = 0; t < nt; ++t)
-    LSH_REQUIRE_C(!failed[t], "compressed horizon block did not decode to the azimuth count");
+    LSH_REQUIRE(!failed[t], "compressed horizon block did not decode to the azimuth count");
   return LSH_OK;
 }
 
 int lsh_cbin_complete(const uint8_t* payload, int64_t bytes, int32_t pixels, int32_t azimuths,
                       int32_t* complete) {
-  LSH_REQUIRE_C(payload && complete && bytes >= 0 && pixels > 0 && azimuths > 0,
+  LSH_REQUIRE(payload && complete && bytes >= 0 && pixels > 0 && azimuths > 0,
                 "invalid arguments");
   // HorizonTileStore.is_complete (file_format.py:335-356): every block length in
   // [1, 2N], every block present, nothing after the last block.

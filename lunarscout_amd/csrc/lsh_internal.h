@@ -19,6 +19,28 @@ constexpr int kSegmentFields = 18; // SEGMENT_FIELDS (contract.py:33-38)
 // Thread-local error text set by every failing entry point.
 void set_error(const std::string& message);
 
+}  // namespace lsh
+
+// Status helpers of the C ABI entry points.
+#define LSH_CHECK_HIP(expr, what)                                              \
+  do {                                                                         \
+    hipError_t _e = (expr);                                                    \
+    if (_e != hipSuccess) {                                                    \
+      lsh::set_error(std::string(what) + ": " + hipGetErrorString(_e));        \
+      return _e == hipErrorOutOfMemory ? LSH_ENOMEM : LSH_EDEVICE;             \
+    }                                                                          \
+  } while (0)
+
+#define LSH_REQUIRE(cond, message) \
+  do {                             \
+    if (!(cond)) {                 \
+      lsh::set_error(message);     \
+      return LSH_EINVAL;           \
+    }                              \
+  } while (0)
+
+namespace lsh {
+
 struct Dem {
   bool live = false;
   int32_t width = 0, height = 0;
@@ -100,6 +122,62 @@ struct SegmentArgs {
 };
 
 hipError_t launch_segments(const SegmentArgs& args, hipStream_t stream);
+
+// SegmentDem of a registered DEM: map resolution and ray limit
+// min(maximum distance, 1.2 * min(width, height) * resolution) (geometry.py:452-462).
+SegmentDem segment_dem(const Dem& dem, double maximum_distance_m);
+Dem* find_dem(Session* s, int32_t id);
+int select_device(Session* s);
+
+// Batches: a site is one DEM list (primary first) with its observer height; a
+// batch tile (lsh_batch_tile) names its site and origin.
+struct SegmentSite {
+  int32_t D;
+  double observer_elevation;
+  SegmentDem dem[kMaxDems];
+};
+
+struct SegmentBatchArgs {
+  const SegmentSite* sites;    // device
+  const lsh_batch_tile* tiles; // device [tile_count]
+  float* segments;             // device, tile slot i at segments + i * slot_floats
+  int64_t slot_floats;
+  int32_t tile_count, N, C, count, subpatch;
+  double maximum_distance;
+};
+
+hipError_t launch_segments_batch(const SegmentBatchArgs& args, hipStream_t stream);
+
+struct MarchSite {
+  int32_t D;
+  float observer_elevation;
+  MarchDem dem[kMaxDems];
+};
+
+struct BatchMarchArgs {
+  const MarchSite* sites;       // device
+  const lsh_batch_tile* tiles;  // device [tile_count]
+  const float* segments;        // device, tile slot i at segments + i * slot_floats
+  int64_t slot_floats;
+  float* output;                // device [tile][16384][N]
+  unsigned int* queues;         // device [8] work counters, zero at launch
+  int32_t tile_count, N, C, subpatch, flags;
+};
+
+// Kernel shapes of the batched march (pixel block kBX x kBY per wavefront,
+// kAZ azimuths per workgroup); index 0 is the plain kernel's shape.
+struct BatchShape {
+  int32_t block_x, block_y, azimuths;
+};
+int batch_shape_count();
+BatchShape batch_shape(int config);
+// Launches the persistent batched march with `workgroups` workgroups (a
+// multiple of 8); *resident receives the occupancy-derived default when
+// workgroups <= 0 is passed.
+hipError_t launch_march_batch(const BatchMarchArgs& args, int config, int workgroups,
+                              hipStream_t stream);
+int batch_default_workgroups(int config, int cu_count);
+hipError_t launch_reset_queues(unsigned int* queues, hipStream_t stream);
 
 struct LightArgs {
   const float* elevation;  // device level 0 of the DEM the tiles belong to

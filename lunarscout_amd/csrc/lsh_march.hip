@@ -44,6 +44,72 @@ constexpr int kAzimuthsPerGroup = 4;  // wavefronts per workgroup
 #define LSH_MARCH_WAVES_PER_EU 4
 #endif
 
+// One ray fan of one pixel at one azimuth over every DEM pass, written to
+// output[pixel * N + az]. `dems` are the tile's D DEMs (kernel arguments or
+// the batch site table); s_off / s_w / s_h their level tables in LDS.
+template <class V, bool kCap>
+__device__ __attribute__((always_inline)) void march_pixel(
+    const MarchDem* dems, const int32_t (*s_off)[kMaxLevels], const int32_t (*s_w)[kMaxLevels],
+    const int32_t (*s_h)[kMaxLevels], int D, const float* segments, int N, int C,
+    int tile_column, int tile_row, int tile_width, int subpatch, float observer_elevation,
+    int flags, int column, int row, int az, float* output, int32_t* winners,
+    unsigned long long* counters) {
+  const int pixel = row * tile_width + column;
+  const int64_t out_index = static_cast<int64_t>(pixel) * N + az;
+  const MarchDem& primary = dems[0];
+  if (tile_column + column >= primary.width || tile_row + row >= primary.height) {
+    output[out_index] = -INFINITY;  // padded pixels keep the sentinel
+    if (winners) winners[out_index] = -1;
+    return;
+  }
+  const PixelSelection sel = lsh_select(column, row, tile_column, tile_row, tile_width, subpatch,
+                                        primary.width, primary.height);
+  const V oz = lsh_observer_terrain<V>(primary.level0, primary.width, primary.height,
+                                       tile_column + column, tile_row + row) +
+               static_cast<V>(observer_elevation);
+  V current = V(-1e30);
+  int32_t best = -1;
+  MarchCounters cnt{0, 0, 0};
+  const int64_t seg_stride = static_cast<int64_t>(D) * kSegmentFields;
+  const float* seg_az = segments + static_cast<int64_t>(az) * C * seg_stride;
+  for (int pass = 0; pass < D; ++pass) {
+    const MarchDem& d = dems[pass];
+    const float* p = seg_az + pass * kSegmentFields;
+    const PassSegment<double> g = lsh_interpolate<double>(
+        p + sel.i00 * seg_stride, p + sel.i10 * seg_stride, p + sel.i01 * seg_stride,
+        p + sel.i11 * seg_stride, sel, column, row,
+        primary.column_resolution / d.column_resolution);
+    MarchView view{d.level0, d.mips, s_off[pass], s_w[pass], s_h[pass], d.width, d.height,
+                   d.level_count, d.map_resolution, d.radius, d.all_valid};
+    lsh_march_pass<double, V, kCap, true>(g, view, oz, pass, current, best, cnt);
+  }
+  float result = current > V(-1e29) ? static_cast<float>(current) : -INFINITY;
+  if (flags & LSH_HORIZON_DEGREES) {
+    // HorizonBuffers.degrees (contract.py:362-371)
+    result = isinf(result) ? result : atanf(result) * (180.0f / 3.14159265358979323846f);
+  }
+  output[out_index] = result;
+  if (winners) winners[out_index] = best;
+  if (counters) {
+    // counters[3]: 64 x the wavefront's longest lane (iterations the SIMD issues)
+    unsigned long long a = cnt.samples, b = cnt.tests, c = cnt.steps;
+    unsigned m = cnt.steps;
+    for (int o = 32; o > 0; o >>= 1) {
+      a += __shfl_down(a, o);
+      b += __shfl_down(b, o);
+      c += __shfl_down(c, o);
+      unsigned other = __shfl_xor(m, o);
+      m = other > m ? other : m;
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&counters[0], a);
+      atomicAdd(&counters[1], b);
+      atomicAdd(&counters[2], c);
+      atomicAdd(&counters[3], 64ull * m);
+    }
+  }
+}
+
 template <class V, bool kCap>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LSH_MARCH_WAVES_PER_EU)))
 march_kernel(MarchArgs args) {
@@ -75,61 +141,83 @@ march_kernel(MarchArgs args) {
   const int column = (pblock % nbx) * kBlockPixels + lane % kBlockPixels;
   const int row = (pblock / nbx) * kBlockPixels + lane / kBlockPixels;
   if (az >= args.N || column >= args.tile_width || row >= args.tile_height) return;
-  const int pixel = row * args.tile_width + column;
-  const int64_t out_index = static_cast<int64_t>(pixel) * args.N + az;
-  const MarchDem& primary = args.dem[0];
-  if (args.tile_column + column >= primary.width || args.tile_row + row >= primary.height) {
-    args.output[out_index] = -INFINITY;  // padded pixels keep the sentinel
-    if (args.winners) args.winners[out_index] = -1;
-    return;
-  }
-  const PixelSelection sel = lsh_select(column, row, args.tile_column, args.tile_row,
-                                        args.tile_width, args.subpatch, primary.width,
-                                        primary.height);
-  const V oz = lsh_observer_terrain<V>(primary.level0, primary.width, primary.height,
-                                       args.tile_column + column, args.tile_row + row) +
-               static_cast<V>(args.observer_elevation);
-  V current = V(-1e30);
-  int32_t best = -1;
-  MarchCounters cnt{0, 0, 0};
-  const int64_t seg_stride = static_cast<int64_t>(args.D) * kSegmentFields;
-  const float* seg_az = args.segments + static_cast<int64_t>(az) * args.C * seg_stride;
-  for (int pass = 0; pass < args.D; ++pass) {
-    const MarchDem& d = args.dem[pass];
-    const float* p = seg_az + pass * kSegmentFields;
-    const PassSegment<double> g = lsh_interpolate<double>(p + sel.i00 * seg_stride, p + sel.i10 * seg_stride,
-                                          p + sel.i01 * seg_stride, p + sel.i11 * seg_stride,
-                                          sel, column, row,
-                                          primary.column_resolution / d.column_resolution);
-    MarchView view{d.level0, d.mips, s_off[pass], s_w[pass], s_h[pass], d.width, d.height,
-                   d.level_count, d.map_resolution, d.radius, d.all_valid};
-    lsh_march_pass<double, V, kCap, true>(g, view, oz, pass, current, best, cnt);
-  }
-  float result = current > V(-1e29) ? static_cast<float>(current) : -INFINITY;
-  if (args.flags & LSH_HORIZON_DEGREES) {
-    // HorizonBuffers.degrees (contract.py:362-371)
-    result = isinf(result) ? result : atanf(result) * (180.0f / 3.14159265358979323846f);
-  }
-  args.output[out_index] = result;
-  if (args.winners) args.winners[out_index] = best;
-  if (args.counters) {
-    // counters[3]: 64 x the wavefront's longest lane (iterations the SIMD issues)
-    unsigned long long a = cnt.samples, b = cnt.tests, c = cnt.steps;
-    unsigned m = cnt.steps;
-    for (int o = 32; o > 0; o >>= 1) {
-      a += __shfl_down(a, o);
-      b += __shfl_down(b, o);
-      c += __shfl_down(c, o);
-      unsigned other = __shfl_xor(m, o);
-      m = other > m ? other : m;
+  march_pixel<V, kCap>(args.dem, s_off, s_w, s_h, args.D, args.segments, args.N, args.C,
+                       args.tile_column, args.tile_row, args.tile_width, args.subpatch,
+                       args.observer_elevation, args.flags, column, row, az, args.output,
+                       args.winners, args.counters);
+}
+
+// The persistent batched march. Work items are (azimuth group, tile, pixel
+// block) in that order; XCD k owns queue k, the k-th eighth of the items (one
+// angular sector of every tile in the batch, so its L2 holds one sector of each
+// DEM), and steals from the other queues once its own is drained. Thread 0
+// takes an item with one atomic add and hands it to the workgroup through LDS
+// (double-buffered by iteration parity, so no second barrier is needed); a
+// workgroup leaves when every queue is exhausted, which each reaches since the
+// counters only grow.
+template <int kBX, int kBY, int kAZ>
+__global__ void __launch_bounds__(64 * kAZ) __attribute__((amdgpu_waves_per_eu(LSH_MARCH_WAVES_PER_EU)))
+march_batch_kernel(BatchMarchArgs a) {
+  static_assert(kBX * kBY == 64, "a wavefront holds 64 pixels");
+  __shared__ int32_t s_off[kMaxDems][kMaxLevels];
+  __shared__ int32_t s_w[kMaxDems][kMaxLevels];
+  __shared__ int32_t s_h[kMaxDems][kMaxLevels];
+  __shared__ int32_t s_item[2];
+  constexpr int kNbx = 128 / kBX, kNby = 128 / kBY, kBlocks = kNbx * kNby;
+  const int groups = (a.N + kAZ - 1) / kAZ;
+  const int64_t per_group = static_cast<int64_t>(a.tile_count) * kBlocks;
+  const int64_t total = per_group * groups;
+  const int xcd = blockIdx.x % 8;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int loaded_site = -1;
+  for (int iteration = 0;; ++iteration) {
+    const int parity = iteration & 1;
+    if (threadIdx.x == 0) {
+      int64_t item = -1;
+      for (int k = 0; k < 8 && item < 0; ++k) {
+        const int q = (xcd + k) & 7;
+        const int64_t begin = total * q / 8, end = total * (q + 1) / 8;
+        if (begin >= end) continue;
+        const unsigned got = atomicAdd(&a.queues[q], 1u);
+        if (begin + static_cast<int64_t>(got) < end) item = begin + got;
+      }
+      s_item[parity] = static_cast<int32_t>(item);
     }
-    if ((threadIdx.x & 63) == 0) {
-      atomicAdd(&args.counters[0], a);
-      atomicAdd(&args.counters[1], b);
-      atomicAdd(&args.counters[2], c);
-      atomicAdd(&args.counters[3], 64ull * m);
+    __syncthreads();
+    const int item = __builtin_amdgcn_readfirstlane(s_item[parity]);
+    if (item < 0) break;
+    const int group = static_cast<int>(item / per_group);
+    const int rest = static_cast<int>(item % per_group);
+    const int tile = rest / kBlocks, pblock = rest % kBlocks;
+    const lsh_batch_tile t = a.tiles[tile];
+    const int site = __builtin_amdgcn_readfirstlane(t.site);
+    const MarchSite& ms = a.sites[site];
+    if (site != loaded_site) {  // every wave left the previous item at the barrier above
+      for (int i = threadIdx.x; i < kMaxDems * kMaxLevels; i += blockDim.x) {
+        const int dd = i / kMaxLevels, l = i % kMaxLevels;
+        s_off[dd][l] = ms.dem[dd].level_offset[l];
+        s_w[dd][l] = ms.dem[dd].level_width[l];
+        s_h[dd][l] = ms.dem[dd].level_height[l];
+      }
+      __syncthreads();
+      loaded_site = site;
+    }
+    const int az = group * kAZ + wave;
+    const int column = (pblock % kNbx) * kBX + lane % kBX;
+    const int row = (pblock / kNbx) * kBY + lane / kBX;
+    if (az < a.N) {
+      march_pixel<double, false>(ms.dem, s_off, s_w, s_h, ms.D,
+                                 a.segments + tile * a.slot_floats, a.N, a.C, t.tile_column,
+                                 t.tile_row, 128, a.subpatch, ms.observer_elevation, a.flags,
+                                 column, row, az,
+                                 a.output + static_cast<int64_t>(tile) * 128 * 128 * a.N,
+                                 nullptr, nullptr);
     }
   }
+}
+
+__global__ void reset_queues_kernel(unsigned int* queues) {
+  if (threadIdx.x < 8) queues[threadIdx.x] = 0u;
 }
 
 }  // namespace
@@ -166,6 +254,58 @@ void launch_march_values(MarchArgs args, dim3 grid, dim3 block, hipStream_t stre
     hipLaunchKernelGGL((march_kernel<V, true>), grid, block, 0, stream, args);
   else
     hipLaunchKernelGGL((march_kernel<V, false>), grid, block, 0, stream, args);
+}
+
+namespace {
+
+constexpr BatchShape kShapes[] = {{8, 8, 4}, {8, 8, 2}, {16, 4, 4}, {4, 16, 4},
+                                  {16, 4, 2}, {4, 16, 2}, {8, 8, 1}};
+
+template <int kBX, int kBY, int kAZ>
+const void* batch_kernel_symbol() {
+  return reinterpret_cast<const void*>(&march_batch_kernel<kBX, kBY, kAZ>);
+}
+
+const void* batch_kernel(int config) {
+  switch (config) {
+    case 0: return batch_kernel_symbol<8, 8, 4>();
+    case 1: return batch_kernel_symbol<8, 8, 2>();
+    case 2: return batch_kernel_symbol<16, 4, 4>();
+    case 3: return batch_kernel_symbol<4, 16, 4>();
+    case 4: return batch_kernel_symbol<16, 4, 2>();
+    case 5: return batch_kernel_symbol<4, 16, 2>();
+    default: return batch_kernel_symbol<8, 8, 1>();
+  }
+}
+
+}  // namespace
+
+int batch_shape_count() { return static_cast<int>(sizeof(kShapes) / sizeof(kShapes[0])); }
+
+BatchShape batch_shape(int config) { return kShapes[config]; }
+
+int batch_default_workgroups(int config, int cu_count) {
+  int per_cu = 0;
+  const BatchShape shape = kShapes[config];
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel(config),
+                                                   64 * shape.azimuths, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 16 / shape.azimuths;
+  const int wgs = per_cu * (cu_count > 0 ? cu_count : 256);
+  return wgs < 8 ? 8 : wgs - wgs % 8;
+}
+
+hipError_t launch_march_batch(const BatchMarchArgs& args, int config, int workgroups,
+                              hipStream_t stream) {
+  const BatchShape shape = kShapes[config];
+  void* params[] = {const_cast<BatchMarchArgs*>(&args)};
+  return hipLaunchKernel(batch_kernel(config), dim3(workgroups), dim3(64 * shape.azimuths),
+                         params, 0, stream);
+}
+
+hipError_t launch_reset_queues(unsigned int* queues, hipStream_t stream) {
+  hipLaunchKernelGGL(reset_queues_kernel, dim3(1), dim3(64), 0, stream, queues);
+  return hipGetLastError();
 }
 
 hipError_t launch_march(const MarchArgs& args, hipStream_t stream) {

@@ -256,17 +256,18 @@ __device__ int clamp_center(int requested, int dem_size, int sub) {
 }
 
 // One (center slot, azimuth) job over every DEM (geometry.py:605-663).
-__global__ void __launch_bounds__(128) segments_kernel(SegmentArgs args) {
-  int64_t job = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (job >= static_cast<int64_t>(args.C) * args.N) return;
-  int slot = static_cast<int>(job / args.N);
-  int az = static_cast<int>(job % args.N);
-  const SegmentDem& primary = args.dem[0];
-  int gr = slot / args.count, gc = slot % args.count;
-  int cc = clamp_center(args.tile_column + (gc - 1) * args.subpatch + args.subpatch / 2,
-                        primary.width, args.subpatch);
-  int cr = clamp_center(args.tile_row + (gr - 1) * args.subpatch + args.subpatch / 2,
-                        primary.height, args.subpatch);
+// `dems` holds the D DEMs of the tile (primary first); `out` the tile's
+// [N][C][D][18] segment tensor.
+__device__ __attribute__((always_inline)) void segment_job(
+    const SegmentDem* dems, int D, float* out_tile, int N, int C, int count, int tile_column,
+    int tile_row, int subpatch, double maximum_distance, double observer_elevation, int slot,
+    int az) {
+  const SegmentDem& primary = dems[0];
+  int gr = slot / count, gc = slot % count;
+  int cc = clamp_center(tile_column + (gc - 1) * subpatch + subpatch / 2,
+                        primary.width, subpatch);
+  int cr = clamp_center(tile_row + (gr - 1) * subpatch + subpatch / 2,
+                        primary.height, subpatch);
   // inverse_stereographic (geometry.py:153-174) of the center pixel
   double x = primary.t[0] + primary.t[1] * cc + primary.t[2] * cr;
   double y = primary.t[3] + primary.t[4] * cc + primary.t[5] * cr;
@@ -298,13 +299,13 @@ __global__ void __launch_bounds__(128) segments_kernel(SegmentArgs args) {
   double terrain = q11 * (x2 - tc) * (y2 - tr) + q21 * (tc - x1) * (y2 - tr) +
                    q12 * (x2 - tc) * (tr - y1) + q22 * (tc - x1) * (tr - y1);
   double correction_radius = p[0] + terrain;
-  double radius = correction_radius + args.observer_elevation;
+  double radius = correction_radius + observer_elevation;
   double sl, cl, sn, cn;
   sincos(lat, &sl, &cl);
   sincos(lon, &sn, &cn);
   double obs[3] = {radius * cl * cn, radius * cl * sn, radius * sl};
   // azimuth_direction (geometry.py:228-234): clockwise from north
-  double angle = kPi / 2.0 - az * 2.0 * kPi / args.N;
+  double angle = kPi / 2.0 - az * 2.0 * kPi / N;
   double sa, ca;
   sincos(angle, &sa, &ca);
   double dir[3] = {ca * -sn + sa * (-sl * cn), ca * cn + sa * (-sl * sn), sa * cl};
@@ -314,18 +315,41 @@ __global__ void __launch_bounds__(128) segments_kernel(SegmentArgs args) {
   Sample samples[kSamples];
   double start = 1.0;
   float seg[kSegmentFields];
-  for (int d = 0; d < args.D; ++d) {
-    float* out = args.out + ((static_cast<int64_t>(az) * args.C + slot) * args.D + d) * kSegmentFields;
-    if (!(start < args.maximum_distance)) {
+  for (int d = 0; d < D; ++d) {
+    float* out = out_tile + ((static_cast<int64_t>(az) * C + slot) * D + d) * kSegmentFields;
+    if (!(start < maximum_distance)) {
       for (int f = 0; f < kSegmentFields; ++f) out[f] = 0.0f;
       continue;
     }
-    const SegmentDem& g = args.dem[d];
+    const SegmentDem& g = dems[d];
     int count = build_samples(obs, dir, start, g.ray_limit, g, samples);
     fit_one(samples, count, g.map_resolution, obs, correction_radius, start, seg);
     for (int f = 0; f < kSegmentFields; ++f) out[f] = seg[f];
     start = fmin(g.ray_limit, static_cast<double>(seg[13]) * 1000.0);
   }
+}
+
+
+__global__ void __launch_bounds__(128) segments_kernel(SegmentArgs args) {
+  int64_t job = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (job >= static_cast<int64_t>(args.C) * args.N) return;
+  segment_job(args.dem, args.D, args.out, args.N, args.C, args.count, args.tile_column,
+              args.tile_row, args.subpatch, args.maximum_distance, args.observer_elevation,
+              static_cast<int>(job / args.N), static_cast<int>(job % args.N));
+}
+
+// Every tile of a batch in one launch: blockIdx.y is the tile slot; the tile's
+// site (its DEM list) and origin come from the device tables.
+__global__ void __launch_bounds__(128) segments_batch_kernel(SegmentBatchArgs a) {
+  const int tile = blockIdx.y;
+  if (tile >= a.tile_count) return;
+  int64_t job = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (job >= static_cast<int64_t>(a.C) * a.N) return;
+  const lsh_batch_tile t = a.tiles[tile];
+  const SegmentSite& site = a.sites[t.site];
+  segment_job(site.dem, site.D, a.segments + tile * a.slot_floats, a.N, a.C, a.count,
+              t.tile_column, t.tile_row, a.subpatch, a.maximum_distance,
+              site.observer_elevation, static_cast<int>(job / a.N), static_cast<int>(job % a.N));
 }
 
 }  // namespace
@@ -335,6 +359,15 @@ hipError_t launch_segments(const SegmentArgs& args, hipStream_t stream) {
   int threads = 128;
   int blocks = static_cast<int>((jobs + threads - 1) / threads);
   hipLaunchKernelGGL(segments_kernel, dim3(blocks), dim3(threads), 0, stream, args);
+  return hipGetLastError();
+}
+
+hipError_t launch_segments_batch(const SegmentBatchArgs& args, hipStream_t stream) {
+  int64_t jobs = static_cast<int64_t>(args.C) * args.N;
+  int threads = 128;
+  int blocks = static_cast<int>((jobs + threads - 1) / threads);
+  hipLaunchKernelGGL(segments_batch_kernel, dim3(blocks, args.tile_count), dim3(threads), 0,
+                     stream, args);
   return hipGetLastError();
 }
 

@@ -139,3 +139,45 @@ def site_dems(config: SiteConfig, scale: int = 1) -> list[DemGrid]:
         dems.append(fbm_crater_dem(config.regional_size // scale, config.regional_resolution_m,
                                    config.regional_seed))
     return dems
+
+
+@dataclass(frozen=True)
+class LandingSite:
+    """One landing-site patch of the batched workload."""
+
+    index: int
+    seed: int
+    tile_count: int     # drawn log-uniform in [1, 64]
+    tiles_x: int        # ceil(sqrt(n)) tile columns
+    tiles_y: int        # ceil(n / tiles_x) tile rows
+    resolution_m: float
+    centre_xy: tuple[float, float]
+
+
+def landing_sites(count: int = 48, first_seed: int = 100, max_tiles: int = 64,
+                  offset_km: float = 60.0) -> list[LandingSite]:
+    """The uneven-site workload: tile counts n = round(exp(U[0, ln max_tiles])),
+    patch DEM ceil(sqrt(n)) x ceil(n / ceil(sqrt(n))) tiles, resolution U[5, 20] m,
+    seeds first_seed + i, centres within +-offset_km of the 89.5 S site."""
+    out = []
+    cx, cy = site_xy()
+    for i in range(count):
+        rng = np.random.default_rng(first_seed + i)
+        n = int(round(math.exp(rng.uniform(0.0, math.log(max_tiles)))))
+        tx = int(math.ceil(math.sqrt(n)))
+        ty = int(math.ceil(n / tx))
+        res = float(rng.uniform(5.0, 20.0))
+        dx, dy = rng.uniform(-offset_km * 1000.0, offset_km * 1000.0, 2)
+        out.append(LandingSite(i, first_seed + i, n, tx, ty, res, (cx + dx, cy + dy)))
+    return out
+
+
+def landing_site_dem(site: LandingSite) -> DemGrid:
+    """The site's patch DEM: (tiles_y * 128) x (tiles_x * 128) px centred on the site."""
+    size = 128 * max(site.tiles_x, site.tiles_y)
+    dem = fbm_crater_dem(size, site.resolution_m, site.seed, centre_xy=site.centre_xy)
+    h, w = 128 * site.tiles_y, 128 * site.tiles_x
+    half_w, half_h = w * site.resolution_m / 2.0, h * site.resolution_m / 2.0
+    transform = np.array((site.centre_xy[0] - half_w, site.resolution_m, 0.0,
+                          site.centre_xy[1] + half_h, 0.0, -site.resolution_m), dtype=np.float64)
+    return DemGrid(np.ascontiguousarray(dem.elevation_m[:h, :w]), transform, dem.projection)

@@ -54,10 +54,16 @@ def test_generate_horizons_writes_resumes_and_overwrites(tmp_path):
         assert path is not None and path.suffix == ".cbin" and path.name.endswith("_015.cbin")
         got = tiles.read_horizon_tile(path)
         want = session.horizon_tile(dems, tile_column=tx, tile_row=ty, observer_elevation_m=1.5,
-                                    degrees=True).reshape(128, 128, 1440)
-        region, expected = got[:vh, :vw], np.clip(want[:vh, :vw], -50.0, 50.0)
-        assert np.max(np.abs(region - expected)) <= 0.5 * QUANTUM + 8e-6
+                                    degrees=True)
+        # the tile is exactly the codec's image of the session's degrees
+        normalized = tiles.normalize_tile(want, valid_width=vw, valid_height=vh)
+        expected = tiles.decode_cbin(tiles.encode_cbin(normalized)).reshape(128, 128, 1440)
+        np.testing.assert_array_equal(got, expected)
         assert np.all(got[vh:] == np.float32(-50.0)) and np.all(got[:, vw:] == np.float32(-50.0))
+        # away from the +-16384-step delta clamp the codec is within half a step
+        smooth = np.abs(np.diff(np.clip(normalized, -50, 50), axis=1)).max(axis=1) < 20.0
+        err = np.abs(expected.reshape(16384, 1440) - np.clip(normalized, -50, 50))[smooth]
+        assert err.size == 0 or err.max() <= 0.5 * QUANTUM + 8e-6
     session.close()
     # resume: nothing is regenerated
     mtimes = {p: p.stat().st_mtime_ns for p in out.rglob("*.cbin")}
