@@ -36,6 +36,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "horizon tiles/s + lightmap px·times/s, 1 MI355X; batched vs plain at ≤0.01° err"
 HBM_PEAK_GBS = 8000.0
+HORIZON_DEGREES_FLAG = 1
 AZIMUTHS = 1440
 PIXELS = 128 * 128
 
@@ -56,6 +57,11 @@ def parse_args():
                    help="tiles re-run with traversal counters for the roofline (0 skips)")
     p.add_argument("--no-probes", action="store_true",
                    help="skip the copy / gather peak probes")
+    p.add_argument("--batch-tiles", type=int, default=64,
+                   help="tiles of the 48-site workload timed plain and batched (0 skips)")
+    p.add_argument("--batch-size", type=int, default=32, help="tiles per batched launch")
+    p.add_argument("--no-year-lighting", dest="year_lighting", action="store_false",
+                   help="skip the 365-day lighting pass over resident horizons")
     return p.parse_args()
 
 
@@ -161,6 +167,9 @@ def main() -> int:
         bytes_alg = 16.0 * (S + T) + 256.0 * rays * passes + 4.0 * rays
         achieved = bytes_alg / (march_ms / 1000.0) / 1e9
     traffic = _traffic_from_profile()
+    long_lighting = None
+    if args.year_lighting:
+        long_lighting = _year_lighting(session, lighting, primary, horizons, tiles[:per_step])
     peaks = None
     if not args.no_probes:
         from lunarscout_amd.profiling import probe_peaks
@@ -198,6 +207,7 @@ def main() -> int:
             "unit": "px·times/s (sun + earth + PSR vectors)",
             "kernel_ms_per_batch": round(lighting_ms, 3),
             "times": {"sun": len(sun), "earth": len(earth), "psr": len(psr_vectors)},
+            "year": long_lighting,
         },
         "roofline": {
             "kernel": "march_kernel",
@@ -213,10 +223,115 @@ def main() -> int:
                                   if peaks and achieved is not None else None),
         },
     }
+    if args.batch_tiles > 0:
+        result["batched"] = _batched_vs_plain(session, dems[1], args.batch_tiles, args.batch_size)
     if args.cpu_sample_pixels > 0:
         result["cpu_baseline"] = _cpu_baseline(dems, sun, args.cpu_sample_pixels)
     print(json.dumps(result), flush=True)
     return 0
+
+
+def _year_lighting(session, lighting, primary, horizons, batch):
+    """Long-duration lighting from stored horizons: 4380 Sun and 4380 Earth
+    vectors (2-hour steps over 365 days, seeds 5 and 6) over the resident tiles,
+    per-time sun-fraction bytes + Earth visibility + PSR in one streamed launch."""
+    from lunarscout_amd import synthetic
+    from lunarscout_amd._lib import DeviceBuffer, Event
+    from lunarscout_amd.lighting import reduce_sun_vectors_for_psr
+
+    sun = synthetic.sun_vectors(primary, 4380, 2.0, seed=5)
+    earth = synthetic.earth_vectors(4380, 2.0, seed=6)
+    psr_vectors, _ = reduce_sun_vectors_for_psr(primary, sun)
+    bufs = [DeviceBuffer.from_host(session, v) for v in (sun, earth, psr_vectors)]
+    kwargs = dict(sun=bufs[0], earth=bufs[1], psr_vectors=bufs[2], sun_bytes=True, sun_mean=True,
+                  earth_visible=True, psr=True)
+    outputs = lighting.lighting_tiles(primary, batch, horizons, **kwargs)  # warm-up
+    e0 = Event(session).record()
+    outputs = lighting.lighting_tiles(primary, batch, horizons, outputs=outputs, **kwargs)
+    e1 = Event(session).record()
+    ms = e0.elapsed_ms(e1)
+    outputs.free()
+    for b in bufs:
+        b.free()
+    px_times = len(batch) * PIXELS * (len(sun) + len(earth) + len(psr_vectors))
+    return {"workload": f"{len(batch)} resident full-site tiles x (4380 Sun + 4380 Earth + "
+                        f"{len(psr_vectors)} PSR vectors), 2-hour steps over 365 days",
+            "value": round(px_times / (ms / 1000.0), 1), "unit": "px·times/s",
+            "kernel_ms": round(ms, 3)}
+
+
+def _batched_vs_plain(session, regional, tile_count, batch_size):
+    """48 landing sites sharing the regional DEM: plain per-tile launches vs the
+    persistent batched launch (hipGraph, autotuned shape), same tiles, same run."""
+    from lunarscout_amd import synthetic
+    from lunarscout_amd._lib import DeviceBuffer, Event
+    from lunarscout_amd.batch import HorizonBatch, Site, autotune
+
+    t0 = time.time()
+    sites = synthetic.landing_sites()
+    site_dems = [synthetic.landing_site_dem(s) for s in sites]
+    work = []  # the first n tiles (row-major) of every patch
+    for i, s in enumerate(sites):
+        for k in range(s.tile_count):
+            work.append((i, (k % s.tiles_x) * 128, (k // s.tiles_x) * 128))
+    pick = np.linspace(0, len(work) - 1, min(tile_count, len(work))).astype(int)
+    sample = [work[i] for i in pick]
+    batch_sites = [Site((d, regional)) for d in site_dems]
+    log(f"48-site workload: {len(work)} tiles, timing {len(sample)} ({time.time() - t0:.1f}s setup)")
+    batch = HorizonBatch(session, batch_sites, max_tiles=batch_size)
+    out = DeviceBuffer.empty(session, (batch_size, PIXELS, AZIMUTHS), np.float32)
+    tune = autotune(batch, sample[: min(8, len(sample))], out, key="landing-sites-48",
+                    use_cache=False)
+    log(f"autotune: shape {tune['config']} workgroups {tune['workgroups']}")
+    # plain: one segments + one march launch per tile
+    segments = None
+    plain_out = DeviceBuffer.empty(session, (PIXELS, AZIMUTHS), np.float32)
+    session.synchronize()
+    e0 = Event(session).record()
+    for site, c, r in sample:
+        dl = [site_dems[site], regional]
+        segments = session.segments_device(dl, tile_column=c, tile_row=r, out=segments)
+        session.march_device(dl, segments, tile_column=c, tile_row=r, flags=HORIZON_DEGREES_FLAG,
+                             output=plain_out)
+    e1 = Event(session).record()
+    session.synchronize()
+    plain_ms = e0.elapsed_ms(e1)
+    log(f"plain: {plain_ms / len(sample):.1f} ms/tile")
+    # batched: persistent launches of batch_size tiles, graph replay
+    e0 = Event(session).record()
+    for start in range(0, len(sample), batch_size):
+        chunk = sample[start:start + batch_size]
+        batch.run(chunk, out, config=tune["config"], workgroups=tune["workgroups"], degrees=True,
+                  use_graph=True)
+    e1 = Event(session).record()
+    session.synchronize()
+    batched_ms = e0.elapsed_ms(e1)
+    log(f"batched: {batched_ms / len(sample):.1f} ms/tile")
+    # the last chunk's first tile, plain, must equal the batched result bit for bit
+    site, c, r = sample[(len(sample) - 1) // batch_size * batch_size]
+    dl = [site_dems[site], regional]
+    segments = session.segments_device(dl, tile_column=c, tile_row=r, out=segments)
+    session.march_device(dl, segments, tile_column=c, tile_row=r, flags=HORIZON_DEGREES_FLAG,
+                         output=plain_out)
+    a, b = plain_out.to_host(), _slot_view(out, 0).to_host()
+    identical = bool(np.all((a == b) | (np.isneginf(a) & np.isneginf(b))))
+    batch.close()
+    out.free()
+    plain_out.free()
+    return {
+        "workload": "48 landing-site patches (tile counts log-uniform 1..64, 5-20 m/px, seeds "
+                    "100-147, centres within +-60 km) sharing the 4096^2 @60 m regional DEM; "
+                    f"{len(work)} tiles, {len(sample)} timed (evenly spaced over the sites)",
+        "plain_tiles_per_s": round(len(sample) / (plain_ms / 1000.0), 4),
+        "batched_tiles_per_s": round(len(sample) / (batched_ms / 1000.0), 4),
+        "speedup": round(plain_ms / batched_ms, 4),
+        "batch_size": batch_size,
+        "autotuned": {"shape": tune["config"], "workgroups": tune["workgroups"],
+                      "trials": [(t["shape"], t["workgroups"], round(t["ms"], 1))
+                                 for t in tune["trials"]]},
+        "bit_identical_to_plain": identical,
+        "max_err_deg_vs_plain": 0.0 if identical else None,
+    }
 
 
 def _slot_view(buffer, slot):
