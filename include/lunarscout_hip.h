@@ -148,6 +148,44 @@ int lsh_horizon_tile_full(lsh_session* session, const int32_t* dem_ids, int32_t 
                           double maximum_distance_m, float observer_elevation_m,
                           int32_t flags, float* output, int64_t* counters);
 
+/* Forward march of one 128x128 tile (as lsh_horizon_tile, subpatch 8, float64
+ * heights) that also stores, per (pixel, azimuth), the int32 index of the
+ * sample that set the horizon: pass << 24 | level-0 sample index within that
+ * pass, or -1 when no sample exists. No reference counterpart: it feeds the
+ * adjoint (the reference has no differentiation).
+ * output float32 [16384][N] (device), winners int32 [16384][N] (device). */
+int lsh_horizon_forward(lsh_session* session, const int32_t* dem_ids, int32_t dem_count,
+                        const float* segments, int32_t azimuth_count, int32_t tile_column,
+                        int32_t tile_row, float observer_elevation_m, int32_t flags,
+                        float* output, int32_t* winners);
+
+/* Smoothed sun-fraction loss of horizon tiles and its gradient with respect to
+ * the horizons (degrees): per pixel sum over Sun vectors of the 16-slice disk
+ * fraction (lightmap.py:32-73) with each slice clamp rounded over
+ * +-smoothing_deg (C1). loss (device, nullable) float64 [tile][16384];
+ * grad_horizon (device, nullable) float32 [tile][16384][N]. Inputs as
+ * lsh_lighting_tiles (pixel-major horizons). smoothing_deg must not exceed the
+ * narrowest slice half-height (0.094 deg for the reference disk). */
+int lsh_sun_loss(lsh_session* session, int32_t dem_id, const lsh_tile* tiles,
+                 int32_t tile_count, const float* horizons, int32_t azimuth_count,
+                 const double* sun, int32_t sun_count, double sun_half_angle_deg,
+                 double smoothing_deg, double* loss, float* grad_horizon);
+
+/* Adjoint of lsh_horizon_forward for one tile: re-marches every ray whose
+ * grad_horizon (float32 [16384][N], d loss / d horizon degrees) is non-zero to
+ * its winning sample and adds d loss / d elevation to the four bilinear nodes of
+ * that sample (in its DEM) and to the observer-terrain nodes of the pixel (in
+ * the primary DEM). grad_fixed: host array of dem_count device int64 buffers
+ * [height][width], accumulated as round(value * fixed_scale) with integer
+ * atomics (deterministic). sample_cells (device, nullable) int32 [16384][N]:
+ * pass << 28 | row << 14 | column of the winning sample's cell, -1 if skipped.
+ * Rays are differentiated with the segment geometry held fixed. */
+int lsh_horizon_adjoint(lsh_session* session, const int32_t* dem_ids, int32_t dem_count,
+                        const float* segments, int32_t azimuth_count, int32_t tile_column,
+                        int32_t tile_row, float observer_elevation_m, const int32_t* winners,
+                        const float* grad_horizon, int64_t* const* grad_fixed,
+                        double fixed_scale, int32_t* sample_cells);
+
 /* Batched horizon generation: many tiles of many sites per launch.
  * It stands in for the reference's per-tile loop (generate_patch_horizons over
  * the bounded pipeline, generator.py:11-36, pipeline.py:127-450) when many

@@ -80,6 +80,9 @@ SIGNATURES = {
     "lsh_cbin_encode": [_vp, _i32, _i32, _vp, _i64, ctypes.POINTER(_i64), _i32],
     "lsh_cbin_decode": [_vp, _i64, _i32, _i32, _vp, _i32],
     "lsh_cbin_complete": [_vp, _i64, _i32, _i32, ctypes.POINTER(_i32)],
+    "lsh_horizon_forward": [_vp, _vp, _i32, _vp, _i32, _i32, _i32, _f32, _i32, _vp, _vp],
+    "lsh_sun_loss": [_vp, _i32, _vp, _i32, _vp, _i32, _vp, _i32, _f64, _f64, _vp, _vp],
+    "lsh_horizon_adjoint": [_vp, _vp, _i32, _vp, _i32, _i32, _i32, _f32, _vp, _vp, _vp, _f64, _vp],
     "lsh_batch_shape_count": [ctypes.POINTER(_i32)],
     "lsh_batch_shape": [_i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
     "lsh_batch_create": [_vp, _vp, _i32, _i32, _i32, _f64, ctypes.POINTER(_vp)],

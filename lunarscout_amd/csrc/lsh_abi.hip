@@ -418,24 +418,22 @@ int lsh_segments_tile(lsh_session* session, const int32_t* dem_ids, int32_t dem_
   return LSH_OK;
 }
 
-int lsh_horizon_tile(lsh_session* session, const int32_t* dem_ids, int32_t dem_count,
-                     const float* segments, int32_t azimuth_count, int32_t tile_column,
-                     int32_t tile_row, int32_t tile_width, int32_t tile_height,
-                     int32_t subpatch_size, float observer_elevation_m, int32_t flags,
-                     float* output, int64_t* counters) {
-  Session* s = reinterpret_cast<Session*>(session);
-  LSH_REQUIRE(s && dem_ids && segments && output, "invalid arguments");
+// Validates a tile march and fills MarchArgs (everything but output / winners).
+static int march_args(Session* s, const int32_t* dem_ids, int32_t dem_count,
+                      const float* segments, int32_t azimuth_count, int32_t tile_column,
+                      int32_t tile_row, int32_t tile_width, int32_t tile_height,
+                      int32_t subpatch_size, float observer_elevation_m, int32_t flags,
+                      lsh::MarchArgs* out) {
+  LSH_REQUIRE(s && dem_ids && segments, "invalid arguments");
   LSH_REQUIRE(dem_count >= 1 && dem_count <= lsh::kMaxDems, "dem_count must be 1..4");
   LSH_REQUIRE(azimuth_count > 0, "azimuth_count must be positive");
   LSH_REQUIRE(tile_width > 0 && tile_height > 0 && tile_width <= 128 && tile_height <= 128,
               "tile dimensions must be 1..128");
   LSH_REQUIRE(subpatch_size > 0 && tile_width / subpatch_size + 2 >= 2, "invalid subpatch size");
   LSH_REQUIRE(std::isfinite(observer_elevation_m), "observer elevation must be finite");
-  lsh::MarchArgs a{};
+  lsh::MarchArgs& a = *out;
+  a = lsh::MarchArgs{};
   a.segments = segments;
-  a.output = output;
-  a.winners = nullptr;
-  a.counters = reinterpret_cast<unsigned long long*>(counters);
   a.N = azimuth_count;
   int count = tile_width / subpatch_size + 2;
   a.C = count * count;
@@ -455,8 +453,65 @@ int lsh_horizon_tile(lsh_session* session, const int32_t* dem_ids, int32_t dem_c
   LSH_REQUIRE(tile_column >= 0 && tile_row >= 0 && tile_column < a.dem[0].width &&
                   tile_row < a.dem[0].height,
               "tile origin lies outside the primary DEM");
+  return LSH_OK;
+}
+
+int lsh_horizon_tile(lsh_session* session, const int32_t* dem_ids, int32_t dem_count,
+                     const float* segments, int32_t azimuth_count, int32_t tile_column,
+                     int32_t tile_row, int32_t tile_width, int32_t tile_height,
+                     int32_t subpatch_size, float observer_elevation_m, int32_t flags,
+                     float* output, int64_t* counters) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(output, "output must not be null");
+  lsh::MarchArgs a;
+  if (int r = march_args(s, dem_ids, dem_count, segments, azimuth_count, tile_column, tile_row,
+                         tile_width, tile_height, subpatch_size, observer_elevation_m, flags, &a))
+    return r;
+  a.output = output;
+  a.winners = nullptr;
+  a.counters = reinterpret_cast<unsigned long long*>(counters);
   if (int r = select_device(s)) return r;
   LSH_CHECK_HIP(lsh::launch_march(a, s->stream), "march kernel launch");
+  return LSH_OK;
+}
+
+int lsh_horizon_forward(lsh_session* session, const int32_t* dem_ids, int32_t dem_count,
+                        const float* segments, int32_t azimuth_count, int32_t tile_column,
+                        int32_t tile_row, float observer_elevation_m, int32_t flags,
+                        float* output, int32_t* winners) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(output && winners, "output and winners must not be null");
+  LSH_REQUIRE((flags & ~LSH_HORIZON_DEGREES) == 0, "forward flags: 0 or LSH_HORIZON_DEGREES");
+  lsh::MarchArgs a;
+  if (int r = march_args(s, dem_ids, dem_count, segments, azimuth_count, tile_column, tile_row,
+                         128, 128, 8, observer_elevation_m, flags, &a))
+    return r;
+  a.output = output;
+  a.winners = winners;
+  if (int r = select_device(s)) return r;
+  LSH_CHECK_HIP(lsh::launch_march(a, s->stream), "march kernel launch");
+  return LSH_OK;
+}
+
+int lsh_horizon_adjoint(lsh_session* session, const int32_t* dem_ids, int32_t dem_count,
+                        const float* segments, int32_t azimuth_count, int32_t tile_column,
+                        int32_t tile_row, float observer_elevation_m, const int32_t* winners,
+                        const float* grad_horizon, int64_t* const* grad_fixed,
+                        double fixed_scale, int32_t* sample_cells) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(winners && grad_horizon && grad_fixed, "invalid arguments");
+  LSH_REQUIRE(std::isfinite(fixed_scale) && fixed_scale > 0.0, "fixed_scale must be positive");
+  lsh::MarchArgs a;
+  if (int r = march_args(s, dem_ids, dem_count, segments, azimuth_count, tile_column, tile_row,
+                         128, 128, 8, observer_elevation_m, 0, &a))
+    return r;
+  for (int i = 0; i < dem_count; ++i) LSH_REQUIRE(grad_fixed[i], "a gradient buffer is null");
+  a.winners = const_cast<int32_t*>(winners);
+  unsigned long long* grids[lsh::kMaxDems] = {nullptr, nullptr, nullptr, nullptr};
+  for (int i = 0; i < dem_count; ++i) grids[i] = reinterpret_cast<unsigned long long*>(grad_fixed[i]);
+  if (int r = select_device(s)) return r;
+  LSH_CHECK_HIP(lsh::launch_adjoint(a, grad_horizon, grids, fixed_scale, sample_cells, s->stream),
+                "adjoint kernel launch");
   return LSH_OK;
 }
 
@@ -575,6 +630,34 @@ int lsh_lighting_tiles(lsh_session* session, int32_t dem_id, const lsh_tile* til
   std::lock_guard<std::mutex> lock(s->mutex);
   if (int r = lsh::upload_tiles(s, tiles, tile_count, &a.tiles)) return r;
   LSH_CHECK_HIP(lsh::launch_lighting(a, s->stream), "lighting kernel launch");
+  return LSH_OK;
+}
+
+int lsh_sun_loss(lsh_session* session, int32_t dem_id, const lsh_tile* tiles,
+                 int32_t tile_count, const float* horizons, int32_t azimuth_count,
+                 const double* sun, int32_t sun_count, double sun_half_angle_deg,
+                 double smoothing_deg, double* loss, float* grad_horizon) {
+  Session* s = reinterpret_cast<Session*>(session);
+  Dem* d = find_dem(s, dem_id);
+  LSH_REQUIRE(d, "unknown DEM id");
+  LSH_REQUIRE(loss || grad_horizon, "nothing to compute");
+  LSH_REQUIRE(sun_count >= 1, "sun vectors are required");
+  lsh::LightArgs a;
+  if (int r = light_args(tiles, tile_count, d->width, d->height, horizons, azimuth_count, 0, sun,
+                         sun_count, nullptr, 0, nullptr, 0, sun_half_angle_deg, 0.545, 0,
+                         nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, &a))
+    return r;
+  LSH_REQUIRE(std::isfinite(smoothing_deg) && smoothing_deg > 0.0 &&
+                  2.0 * smoothing_deg <= 2.0 * a.half_circle[0],
+              "smoothing must be positive and at most the narrowest slice half-height");
+  a.elevation = d->level0;
+  std::memcpy(a.geo_transform, d->geo_transform, sizeof(a.geo_transform));
+  std::memcpy(a.projection, d->projection, sizeof(a.projection));
+  if (int r = select_device(s)) return r;
+  std::lock_guard<std::mutex> lock(s->mutex);
+  if (int r = lsh::upload_tiles(s, tiles, tile_count, &a.tiles)) return r;
+  LSH_CHECK_HIP(lsh::launch_sun_loss(a, smoothing_deg, loss, grad_horizon, s->stream),
+                "sun loss kernel launch");
   return LSH_OK;
 }
 

@@ -211,6 +211,11 @@ struct LightArgs {
 hipError_t launch_lighting(const LightArgs& args, hipStream_t stream);
 void run_lighting_host(const LightArgs& args, int threads);
 hipError_t launch_march(const MarchArgs& args, hipStream_t stream);
+hipError_t launch_sun_loss(const LightArgs& light, double smoothing, double* loss, float* grad,
+                           hipStream_t stream);
+hipError_t launch_adjoint(const MarchArgs& march, const float* grad_horizon,
+                          unsigned long long* const* grad_fixed, double fixed_scale,
+                          int32_t* sample_cells, hipStream_t stream);
 hipError_t launch_probe_copy(const float* src, float* dst, int64_t bytes, hipStream_t stream);
 hipError_t launch_probe_gather(const float* window, int32_t width, int32_t height, int32_t taps,
                                int32_t threads, float* out, hipStream_t stream);

@@ -75,6 +75,13 @@ struct MarchCounters {
 struct NoTrace {
   LSH_HD_MEMBER void row(float, float, int, int, int, float, float, float, float, float, float,
                          int) const {}
+  // Called at every level-0 sample of a pass before it updates the running
+  // maximum: sample index, the sample's cell (ix, iy) and clamped bilinear
+  // fractions, h - oz, the true distance (m) and the near-field flag. A tracer
+  // returning true ends the pass there (the adjoint's re-march to its winner).
+  LSH_HD_MEMBER bool sample(int, int, int, double, double, double, double, bool) const {
+    return false;
+  }
 };
 
 LSH_HD bool lsh_valid(float v) {
@@ -333,6 +340,9 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
       const V den = near ? tm : (lxx > V(0) ? lsh_sqrt(lxx) : V(1e-6));
       V slope = num / den;
       if (!taps_valid) slope = V(-1e30);
+      if (trace.sample(sample_index, ix, iy, static_cast<double>(tx), static_cast<double>(ty),
+                       static_cast<double>(hd), static_cast<double>(tm), near))
+        return;
       if (slope > current) {
         current = slope;
         best = (pass << 24) | sample_index;

@@ -107,6 +107,13 @@ class HipSession(_lib.Session):
             self._dems[id(dem)] = _DemEntry(dem, int(out.value))
             return int(out.value)
 
+    def forget(self, dem: DemGrid) -> None:
+        """Release a registered DEM's device copy (a later use registers it again)."""
+        with self._dem_lock:
+            entry = self._dems.pop(id(dem), None)
+        if entry is not None and entry.dem is dem:
+            check(self.lib.lsh_dem_destroy(self.handle, entry.dem_id), "lsh_dem_destroy")
+
     def build_max_pyramid(self, dem: DemGrid) -> PyramidArrays:
         """Factor-four max pyramid built on the GPU (cuda_backend.py:1260-1281)."""
         dem_id = self.dem_id(dem)

@@ -285,6 +285,36 @@ def horizon_tile(pyramids, segment_values: np.ndarray, *, tile_column: int, tile
     return out, counters
 
 
+def horizon_tile_frozen(pyramids, eval_dems, segment_values: np.ndarray, *, tile_column: int,
+                        tile_row: int, pixels, observer_elevation_m: float = 0.0,
+                        threads: int = 0) -> np.ndarray:
+    """Float64 slopes [P, N] with the schedule of ``pyramids`` and the heights of ``eval_dems``.
+
+    The march runs on the base DEMs; every level-0 sample it takes is also
+    evaluated on the same-shaped ``eval_dems`` (and their observer terrain), and
+    the maximum of those slopes is returned -- the derivative convention of the
+    adjoint, whose finite differences this gives.
+    """
+    keep = _Keep()
+    segs = keep.add(np.ascontiguousarray(segment_values, dtype=np.float32))
+    N, D = segs.shape[0], segs.shape[2]
+    arr = (LshoPyramid * D)(*[p.struct(keep) for p in pyramids])
+    grids = [keep.add(np.ascontiguousarray(d.elevation_m, dtype=np.float32)) for d in eval_dems]
+    for g, p in zip(grids, pyramids):
+        if g.shape != (p.dem.height, p.dem.width):
+            raise ValueError("evaluation DEMs must match the march DEMs")
+    ptrs = (_vp * D)(*[g.ctypes.data for g in grids])
+    px = keep.add(np.ascontiguousarray(pixels, dtype=np.int32))
+    out = np.empty((len(px), N), dtype=np.float64)
+    status = lib().lsho_horizon_tile_frozen(
+        arr, ptrs, _i32(D), _vp(segs.ctypes.data), _i32(N), _i32(tile_column), _i32(tile_row),
+        _f64(observer_elevation_m), _vp(px.ctypes.data), _i32(len(px)), _vp(out.ctypes.data),
+        _i32(threads))
+    if status:
+        raise ValueError("invalid frozen-schedule request")
+    return out
+
+
 def sun_fraction(horizon: np.ndarray, azimuth_deg: float, elevation_deg: float,
                  *, float32: bool = True) -> float:
     hz = np.ascontiguousarray(horizon, dtype=np.float32)
@@ -435,3 +465,58 @@ def trace_tile_ray(pyramids, segment_values, *, tile_column: int, tile_row: int,
         _i32(tile_width), _i32(subpatch_size), _f64(observer_elevation_m), _i32(pixel),
         _i32(azimuth), _vp(rows.ctypes.data), _i32(capacity), ctypes.byref(result))
     return result.value, rows[: min(n, capacity)].copy()
+
+
+def _sun_disk(half_angle_deg: float = 0.27):
+    """The 16-slice table as float32 and max_photons (lightmap.py:15-29)."""
+    angle = float(np.float32(half_angle_deg))
+    half = np.array([np.float32(np.sqrt(64.0 - (7.5 - i) ** 2) / 8.0 * angle) for i in range(16)],
+                    dtype=np.float32)
+    total = np.float32(0.0)
+    for v in half:
+        total = np.float32(total + v)
+    return half, np.float32(2.0) * total
+
+
+def _smooth_clamp(x, W, s):
+    value = np.where(x <= -s, 0.0, np.where(x < s, (x + s) ** 2 / (4 * s),
+                     np.where(x <= W - s, x, np.where(x < W + s, W - (W + s - x) ** 2 / (4 * s), W))))
+    return value
+
+
+def smoothed_sun_loss(dem, horizons_deg, vectors, *, tile_y: int, tile_x: int, pixels,
+                      half_angle_deg: float = 0.27, smoothing_deg: float = 0.02):
+    """Per-pixel sum over vectors of the smoothed 16-slice sun fraction, in float64.
+
+    Restates the loss of the adjoint path: the slice walk of _sun_fraction_reference
+    (lightmap.py:32-73) with float32 bin positions, each slice clamp
+    min(max(x, 0), 2c) replaced by its C1 rounding over +-smoothing_deg; the
+    horizons (degrees, [P, N] rows of the given tile-local pixel indices) enter
+    in float64. Pixel frames and az/el come from ``azimuth_elevation``.
+    """
+    half, max_photons = _sun_disk(half_angle_deg)
+    hz = np.asarray(horizons_deg, dtype=np.float64)
+    N = hz.shape[-1]
+    out = np.zeros(len(pixels))
+    for k, p in enumerate(pixels):
+        row, column = tile_y + int(p) // 128, tile_x + int(p) % 128
+        az, el = azimuth_elevation(dem, row, column, vectors)
+        el_f = el.astype(np.float32).astype(np.float64)
+        left_pos = ((az.astype(np.float32) - np.float32(half_angle_deg)) - np.float32(0.125)) * np.float32(4.0)
+        left = np.trunc(left_pos).astype(np.int64)
+        frac = (left_pos - left.astype(np.float32)).astype(np.float32)
+        left = np.where(left < 0, left + N, np.where(left >= N, left - N, left))
+        right = np.where(left + 1 >= N, 0, left + 1)
+        lo = left.copy()
+        photons = np.zeros(len(az))
+        for i in range(16):
+            c = float(half[i])
+            horizon = frac.astype(np.float64) * (hz[k, right] - hz[k, lo]) + hz[k, lo]
+            photons += _smooth_clamp(el_f + c - horizon, 2.0 * c, smoothing_deg)
+            frac = (frac + np.float32(0.135)).astype(np.float32)
+            step = frac >= np.float32(1.0)
+            lo = np.where(step, right, lo)
+            right = np.where(step, np.where(right + 1 >= N, 0, right + 1), right)
+            frac = np.where(step, (frac - np.float32(1.0)).astype(np.float32), frac)
+        out[k] = float(np.sum(photons / float(max_photons)))
+    return out

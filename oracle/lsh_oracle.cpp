@@ -461,6 +461,13 @@ struct MarchOpts {
   int64_t samples;     // level-0 samples (action 4)
   int64_t tests;       // hierarchy bound tests (actions 0, 1, 2)
   int64_t steps;       // every trace row
+  // Frozen-schedule evaluation (the adjoint's convention): when eval_level0 is
+  // set, every level-0 sample the march takes on its own DEM is also evaluated on
+  // eval_level0 (same grid) against eval_observer_z, and eval_current keeps the
+  // maximum of those slopes; the schedule itself is the march's own.
+  const float* eval_level0 = nullptr;
+  double eval_observer_z = 0.0;
+  double eval_current = -1e30;
 };
 
 inline float cell_value(const MarchDem& d, int level, int x, int y) {
@@ -677,6 +684,25 @@ T hierarchy_maximum(const T* sg, const MarchDem& d, T observer_z, T radius, T re
           }
           if (slope > current) current = slope;
         }
+        if (o.eval_level0) {
+          T he = bilinear<T>(o.eval_level0, d.w, d.h, px, py);
+          if (valid_elevation(double(he))) {
+            T ez = T(o.eval_observer_z);
+            T es;
+            if (s < T(0.5)) {
+              es = (he - ez) / true_m;
+            } else {
+              T sq = true_m * true_m;
+              T hd = he - ez;
+              T rs = T(2) * radius + he;
+              rs = rs + ez;
+              T lz = (hd * rs - sq) / (T(2) * (radius + ez));
+              T lxx = sq - lz * lz;
+              es = lz / (lxx > T(0) ? std::sqrt(lxx) : T(1e-6));
+            }
+            if (double(es) > o.eval_current) o.eval_current = double(es);
+          }
+        }
         T mag = std::sqrt(dx * dx + dy * dy);
         T pixel_step = mag > T(1e-6) ? T(1) / mag : T(0.0005);
         T margin = current - slope;
@@ -732,7 +758,8 @@ inline PassGeometry pass_geometry(const lsho_dem& d, bool f32) {
 template <class T>
 T ray_all_passes(const lsho_pyramid* pyrs, int D, const float* segments, int N, int C,
                  int az, int column, int row, int tile_col, int tile_row, int tile_w,
-                 int subpatch, double obs_h, MarchOpts& o) {
+                 int subpatch, double obs_h, MarchOpts& o,
+                 const float* const* eval_level0 = nullptr) {
   const lsho_dem& primary = pyrs[0].dem;
   bool f32 = sizeof(T) == sizeof(float);
   int count = tile_w / subpatch + 2;
@@ -756,8 +783,14 @@ T ray_all_passes(const lsho_pyramid* pyrs, int D, const float* segments, int N, 
   T obs_terrain = bilinear<T>(primary.elevation, primary.width, primary.height,
                               T(tile_col + column), T(tile_row + row));
   T observer_z = obs_terrain + T(f32 ? double(float(obs_h)) : obs_h);
+  if (eval_level0) {
+    o.eval_observer_z = double(bilinear<T>(eval_level0[0], primary.width, primary.height,
+                                           T(tile_col + column), T(tile_row + row))) + obs_h;
+    o.eval_current = -1e30;
+  }
   T running = -std::numeric_limits<T>::infinity();
   for (int p = 0; p < D; ++p) {
+    o.eval_level0 = eval_level0 ? eval_level0[p] : nullptr;
     PassGeometry pg = pass_geometry(pyrs[p].dem, f32);
     T ratio = T(pg0.column_res) / T(pg.column_res);
     T sxl = T(column - cl) * ratio, sxr = T(column - cr) * ratio;
@@ -1195,6 +1228,32 @@ int lsho_horizon_tile_f64(const lsho_pyramid* pyrs, int32_t D, const float* segm
     counters[0] = s_samples;
     counters[1] = s_tests;
     counters[2] = s_steps;
+  }
+  return 0;
+}
+
+// The frozen-schedule evaluation of a tile (see MarchOpts): the march runs on
+// `pyrs`, every level-0 sample is re-evaluated on eval_level0[pass] (float32
+// grids shaped like the passes' DEMs), out[P][N] = the maximum re-evaluated
+// slope (float64), -inf when the ray has no sample. This is the convention of
+// the adjoint (sample positions and culls held at the forward's), used to check
+// it by finite differences.
+int lsho_horizon_tile_frozen(const lsho_pyramid* pyrs, const float* const* eval_level0, int32_t D,
+                             const float* segments, int32_t N, int32_t tile_col,
+                             int32_t tile_row, double obs_h, const int32_t* pixels, int32_t P,
+                             double* out, int32_t nthreads) {
+  if (!pyrs || !eval_level0 || !segments || !pixels || !out || D < 1 || N < 1 || P < 1) return -1;
+  const int count = 128 / 8 + 2, C = count * count;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t job = 0; job < static_cast<int64_t>(P) * N; ++job) {
+    const int k = static_cast<int>(job / N), az = static_cast<int>(job % N);
+    const int column = pixels[k] % 128, row = pixels[k] / 128;
+    MarchOpts o{};
+    o.stable_slope = true;
+    ray_all_passes<double>(pyrs, D, segments, N, C, az, column, row, tile_col, tile_row, 128, 8,
+                           obs_h, o, eval_level0);
+    out[job] = o.eval_current > -1e29 ? o.eval_current : -std::numeric_limits<double>::infinity();
   }
   return 0;
 }

@@ -163,7 +163,7 @@ int kcpu_horizon_tile(const kc_pyramid* pyrs, int32_t D, const float* segments, 
   return 0;
 }
 
-struct RowTrace {
+struct RowTrace : lsh::NoTrace {
   float* rows;
   int cap;
   int* count;
@@ -187,7 +187,7 @@ int kcpu_trace_ray(const kc_pyramid* pyrs, int32_t D, const float* segments, int
   PassTables t = tables(pyrs, D);
   lsh::MarchCounters cnt{0, 0, 0};
   int n = 0;
-  RowTrace tr{rows, cap, &n};
+  RowTrace tr{{}, rows, cap, &n};
   int32_t best = -1;
   *result = ray_mode(flags, pyrs, D, segments, count * count, az, pixel % tile_width,
                      pixel / tile_width, tile_column, tile_row, tile_width, subpatch, obs_h, t,

@@ -1,0 +1,210 @@
+"""The adjoint: dL/dDEM of the smoothed sun-fraction loss.
+
+* the loss and dL/d(horizon) against a pure-PyTorch autograd reference of the
+  same smoothed 16-slice model (float64);
+* dL/dDEM against central finite differences (h = 0.05 m) of the float64
+  oracle under the adjoint's convention -- sample positions and culls held at
+  the forward's (the march's adaptive steps also move samples continuously with
+  the heights; the winning-sample adjoint does not differentiate that): for
+  each probed node the affected pixels (those whose winning sample touches the
+  node, and the pixel the node carries) are re-evaluated by the oracle with the
+  node moved by +-h and +-h/2 and their loss recomputed in float64. A node is
+  tie-free when the two step sizes agree to 1e-4 (no winner switches inside the
+  step); on those the relative error must be <= 1e-3;
+* run-to-run bit-identical gradients (fixed-point scatter), and the
+  torch.autograd.Function wrapper.
+"""
+
+from __future__ import annotations
+
+import functools
+
+import numpy as np
+import pytest
+
+import oracle
+from lunarscout_amd import synthetic
+from lunarscout_amd.geometry import DemGrid
+
+pytestmark = pytest.mark.gpu
+
+N = 720
+
+
+@functools.lru_cache(maxsize=None)
+def _case():
+    from lunarscout_amd.adjoint import SunLossProblem
+
+    dems = [synthetic.fbm_crater_dem(256, 10.0, 7), synthetic.fbm_crater_dem(512, 60.0, 8)]
+    tiles = [(0, 0), (128, 0), (0, 128), (128, 128)]
+    sun = synthetic.sun_vectors(dems[0], 365, 4.0, seed=9)
+    problem = SunLossProblem(dems, tiles, sun, azimuth_count=N)
+    loss = problem.forward()
+    grads, cells = problem.backward(sample_cells=True)
+    return problem, loss, grads, cells
+
+
+def test_loss_and_horizon_gradient_match_a_pytorch_reference():
+    import torch
+
+    problem, loss, _, _ = _case()
+    horizons = problem.horizons()
+    grad_h = problem.grad_horizons()
+    rng = np.random.default_rng(0)
+    half, max_photons = oracle._sun_disk(0.27)
+    s = problem.smoothing_deg
+    for tile_index, (tx, ty) in enumerate(problem.tiles[:2]):
+        pixels = rng.choice(16384, 24, replace=False)
+        for p in pixels:
+            row, column = ty + p // 128, tx + p % 128
+            az, el = oracle.azimuth_elevation(problem.dems[0], row, column, problem.sun)
+            h = torch.tensor(np.maximum(horizons[tile_index, p].astype(np.float64), -90.0),
+                             requires_grad=True)
+            # the slice walk's bins and float32 fractions (index work, no gradient)
+            left_pos = ((az.astype(np.float32) - np.float32(0.27)) - np.float32(0.125)) * np.float32(4)
+            left = np.trunc(left_pos).astype(np.int64)
+            frac = (left_pos - left.astype(np.float32)).astype(np.float32)
+            left = np.where(left < 0, left + N, np.where(left >= N, left - N, left))
+            right = np.where(left + 1 >= N, 0, left + 1)
+            lo = left.copy()
+            el_t = torch.tensor(el.astype(np.float32).astype(np.float64))
+            total = torch.zeros((), dtype=torch.float64)
+            for i in range(16):
+                c = float(half[i])
+                f = torch.tensor(frac.astype(np.float64))
+                horizon = f * (h[torch.tensor(right)] - h[torch.tensor(lo)]) + h[torch.tensor(lo)]
+                x = el_t + c - horizon
+                W = 2.0 * c
+                value = torch.where(x <= -s, torch.zeros_like(x),
+                        torch.where(x < s, (x + s) ** 2 / (4 * s),
+                        torch.where(x <= W - s, x,
+                        torch.where(x < W + s, W - (W + s - x) ** 2 / (4 * s), torch.full_like(x, W)))))
+                total = total + value.sum()
+                frac = (frac + np.float32(0.135)).astype(np.float32)
+                step = frac >= np.float32(1.0)
+                lo = np.where(step, right, lo)
+                right = np.where(step, np.where(right + 1 >= N, 0, right + 1), right)
+                frac = np.where(step, (frac - np.float32(1.0)).astype(np.float32), frac)
+            total = total / float(max_photons)
+            total.backward()
+            got = grad_h[tile_index, p].astype(np.float64)
+            want = h.grad.numpy()
+            scale = max(np.abs(want).max(), 1e-12)
+            assert np.abs(got - want).max() <= 1e-5 * scale + 1e-7, (tile_index, p)
+    assert 0.0 < loss < 1.0
+
+
+def _perturbed(dem: DemGrid, x: int, y: int, delta: float) -> tuple[DemGrid, float]:
+    """The DEM with node (x, y) moved by delta, and the move float32 storage made."""
+    z = dem.elevation_m.copy()
+    z[y, x] = np.float32(np.float64(z[y, x]) + delta)
+    return DemGrid(z, dem.geo_transform, dem.projection), float(np.float64(z[y, x]) -
+                                                               np.float64(dem.elevation_m[y, x]))
+
+
+def _affected(problem, cells, dem_index, x, y):
+    """(tile, pixel) pairs whose loss can change when node (x, y) of the DEM moves."""
+    out = set()
+    for t, (tx, ty) in enumerate(problem.tiles):
+        c = cells[t]
+        valid = c >= 0
+        passes = np.where(valid, c >> 28, -1)
+        cy = (c >> 14) & 0x3FFF
+        cx = c & 0x3FFF
+        hit = valid & (passes == dem_index) & ((cx == x) | (cx == x - 1)) & ((cy == y) | (cy == y - 1))
+        for p in np.nonzero(hit.any(axis=1))[0]:
+            out.add((t, int(p)))
+        if dem_index == 0 and tx <= x < tx + 128 and ty <= y < ty + 128:
+            out.add((t, (y - ty) * 128 + (x - tx)))
+    return sorted(out)
+
+
+def _fd_loss(problem, dems, affected, segments, pyramids):
+    """Float64 loss of the affected pixels: the oracle marches the base DEMs and
+    evaluates every sample on ``dems`` (the adjoint's frozen-schedule convention)."""
+    total = 0.0
+    by_tile: dict[int, list[int]] = {}
+    for t, p in affected:
+        by_tile.setdefault(t, []).append(p)
+    for t, pixels in by_tile.items():
+        tx, ty = problem.tiles[t]
+        slopes = oracle.horizon_tile_frozen(pyramids, dems, segments[t], tile_column=tx,
+                                            tile_row=ty, pixels=np.array(pixels, np.int32))
+        degrees = np.degrees(np.arctan(slopes))
+        degrees[~np.isfinite(slopes)] = -90.0
+        total += oracle.smoothed_sun_loss(dems[0], degrees, problem.sun, tile_y=ty, tile_x=tx,
+                                          pixels=pixels, smoothing_deg=problem.smoothing_deg).sum()
+    return total
+
+
+def test_adjoint_matches_finite_differences_of_the_float64_oracle():
+    problem, _, grads, cells = _case()
+    dems = problem.dems
+    segments = [oracle.segments(dems, tile_column=tx, tile_row=ty, tile_width=128, azimuth_count=N)
+                for tx, ty in problem.tiles]
+    norm = problem.valid_pixels * len(problem.sun)
+    pyramids = [oracle.OraclePyramid(d) for d in dems]
+    rng = np.random.default_rng(1)
+    checked = rejected = 0
+    worst = 0.0
+    for dem_index, count in ((0, 10), (1, 6)):
+        done = 0
+        g = grads[dem_index]
+        h_, w_ = g.shape
+        interior = np.zeros_like(g, dtype=bool)
+        interior[1:h_ - 2, 1:w_ - 2] = True
+        magnitude = np.abs(g) * interior
+        candidates = np.argwhere(magnitude > np.quantile(magnitude[magnitude > 0], 0.5))
+        for y, x in candidates[rng.choice(len(candidates), min(3 * count, len(candidates)), replace=False)]:
+            if done >= count:
+                break
+            affected = _affected(problem, cells, dem_index, int(x), int(y))
+            if not affected or len(affected) > 120:
+                continue
+            fd = []
+            for step in (0.05, 0.025):
+                values, moves = [], []
+                for sign in (1.0, -1.0):
+                    moved = list(dems)
+                    moved[dem_index], actual = _perturbed(dems[dem_index], int(x), int(y), sign * step)
+                    values.append(_fd_loss(problem, moved, affected, segments, pyramids))
+                    moves.append(actual)
+                fd.append((values[0] - values[1]) / (moves[0] - moves[1]) / norm)
+            if abs(fd[0] - fd[1]) > 1e-4 * max(abs(fd[0]), 1e-30):
+                rejected += 1  # a sample or a cull switches inside the step
+                continue
+            rel = abs(g[y, x] - fd[0]) / max(abs(fd[0]), 1e-30)
+            print(f"DEM {dem_index} node ({x},{y}): adjoint {g[y, x]:.6e} fd {fd[0]:.6e} "
+                  f"rel {rel:.2e} ({len(affected)} px)", flush=True)
+            worst = max(worst, rel)
+            checked += 1
+            done += 1
+            assert rel <= 1e-3, (dem_index, x, y)
+    print(f"checked {checked} tie-free nodes, rejected {rejected}, worst rel {worst:.2e}", flush=True)
+    assert checked >= 8
+
+
+def test_gradients_are_bit_identical_run_to_run():
+    problem, _, grads, _ = _case()
+    again = problem.backward()
+    for a, b in zip(grads, again):
+        assert a.tobytes() == b.tobytes()
+
+
+def test_autograd_function_matches_the_problem():
+    import torch
+    from lunarscout_amd.adjoint import HorizonSunLoss, SunLossProblem
+
+    dems = [synthetic.fbm_crater_dem(256, 10.0, 7), synthetic.fbm_crater_dem(512, 60.0, 8)]
+    sun = synthetic.sun_vectors(dems[0], 64, 4.0, seed=9)
+    problem = SunLossProblem(dems, [(128, 128)], sun, azimuth_count=N)
+    z = [torch.tensor(d.elevation_m, requires_grad=True) for d in dems]
+    loss = HorizonSunLoss.apply(problem, *z)
+    loss.backward()
+    reference = SunLossProblem(dems, [(128, 128)], sun, azimuth_count=N)
+    assert abs(reference.forward() - float(loss)) <= 1e-12
+    want = reference.backward()
+    for tensor, w in zip(z, want):
+        np.testing.assert_allclose(tensor.grad.numpy(), w.astype(np.float32), rtol=1e-6, atol=1e-30)
+    problem.close()
+    reference.close()

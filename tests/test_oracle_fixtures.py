@@ -404,3 +404,22 @@ def test_cbin_round_trip_and_structure():
         oracle.cbin_decode(payload[:-1])
     with pytest.raises(ValueError):
         oracle.cbin_decode(payload + b"\x00")
+
+
+def test_frozen_schedule_evaluation_reduces_to_the_method():
+    """horizon_tile_frozen with the march's own DEMs is the float64 method, and
+    raising a pixel's observer node lowers every horizon of that pixel."""
+    from lunarscout_amd import synthetic
+
+    dems = [synthetic.fbm_crater_dem(256, 20.0, 3)]
+    segs = oracle.segments(dems, tile_column=128, tile_row=128, tile_width=128, azimuth_count=32)
+    pyrs = [oracle.OraclePyramid(d) for d in dems]
+    px = np.array([0, 77, 8000, 16383], np.int32)
+    base, _ = oracle.horizon_tile(pyrs, segs, tile_column=128, tile_row=128, pixels=px, float64=True)
+    same = oracle.horizon_tile_frozen(pyrs, dems, segs, tile_column=128, tile_row=128, pixels=px)
+    np.testing.assert_array_equal(same, base)
+    z = dems[0].elevation_m.copy()
+    z[128 + 77 // 128, 128 + 77 % 128] += 1.0  # the observer node of pixel 77
+    moved = [DemGrid(z, dems[0].geo_transform, dems[0].projection)]
+    out = oracle.horizon_tile_frozen(pyrs, moved, segs, tile_column=128, tile_row=128, pixels=px)
+    assert np.all(out[1] < base[1])  # a higher observer sees every horizon lower
