@@ -62,6 +62,8 @@ def parse_args():
     p.add_argument("--batch-size", type=int, default=32, help="tiles per batched launch")
     p.add_argument("--no-year-lighting", dest="year_lighting", action="store_false",
                    help="skip the 365-day lighting pass over resident horizons")
+    p.add_argument("--no-adjoint", dest="adjoint", action="store_false",
+                   help="skip the DEM-sensitivity (adjoint) leg")
     return p.parse_args()
 
 
@@ -225,6 +227,8 @@ def main() -> int:
     }
     if args.batch_tiles > 0:
         result["batched"] = _batched_vs_plain(session, dems[1], args.batch_tiles, args.batch_size)
+    if args.adjoint:
+        result["adjoint"] = _adjoint_leg(session)
     if args.cpu_sample_pixels > 0:
         result["cpu_baseline"] = _cpu_baseline(dems, sun, args.cpu_sample_pixels)
     print(json.dumps(result), flush=True)
@@ -258,6 +262,38 @@ def _year_lighting(session, lighting, primary, horizons, batch):
                         f"{len(psr_vectors)} PSR vectors), 2-hour steps over 365 days",
             "value": round(px_times / (ms / 1000.0), 1), "unit": "px·times/s",
             "kernel_ms": round(ms, 3)}
+
+
+def _adjoint_leg(session):
+    """DEM sensitivity of the mean smoothed sun fraction: 512^2 @10 m (seed 7) +
+    1024^2 @60 m regional (seed 8), 16 tiles x 720 azimuths, 2190 Sun vectors
+    (4-hourly over 365 days, seed 10); forward = march with winners + loss,
+    backward = horizon gradient + re-march adjoint. Warm-up pass first."""
+    from lunarscout_amd import synthetic
+    from lunarscout_amd.adjoint import SunLossProblem
+
+    dems = synthetic.site_dems(synthetic.ADJOINT_SITE)
+    tiles = [(x, y) for y in range(0, 512, 128) for x in range(0, 512, 128)]
+    sun = synthetic.sun_vectors(dems[0], 2190, 4.0, seed=10)
+    problem = SunLossProblem(dems, tiles, sun, azimuth_count=720, session=session)
+    problem.forward()
+    problem.backward()
+    t0 = time.perf_counter()
+    loss = problem.forward()
+    forward_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    grads = problem.backward()
+    backward_s = time.perf_counter() - t0
+    sec = dict(problem.seconds)
+    problem.close()
+    return {
+        "workload": "512^2 @10 m + 1024^2 @60 m, 16 tiles x 720 az, 2190 Sun vectors (4-hourly, 365 d)",
+        "loss": loss,
+        "forward_s": round(forward_s, 4), "backward_s": round(backward_s, 4),
+        "backward_over_forward": round(backward_s / forward_s, 4),
+        "kernel_s": {k: round(v, 4) for k, v in sec.items()},
+        "grad_nonzero_nodes": [int(np.count_nonzero(g)) for g in grads],
+    }
 
 
 def _batched_vs_plain(session, regional, tile_count, batch_size):

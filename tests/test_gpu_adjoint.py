@@ -2,15 +2,17 @@
 
 * the loss and dL/d(horizon) against a pure-PyTorch autograd reference of the
   same smoothed 16-slice model (float64);
-* dL/dDEM against central finite differences (h = 0.05 m) of the float64
-  oracle under the adjoint's convention -- sample positions and culls held at
-  the forward's (the march's adaptive steps also move samples continuously with
-  the heights; the winning-sample adjoint does not differentiate that): for
-  each probed node the affected pixels (those whose winning sample touches the
-  node, and the pixel the node carries) are re-evaluated by the oracle with the
-  node moved by +-h and +-h/2 and their loss recomputed in float64. A node is
-  tie-free when the two step sizes agree to 1e-4 (no winner switches inside the
-  step); on those the relative error must be <= 1e-3;
+* dL/dDEM against central finite differences of the float64 oracle under the
+  adjoint's convention -- sample positions and culls held at the forward's (the
+  march's adaptive steps also move samples continuously with the heights; the
+  winning-sample adjoint does not differentiate that): for each probed node the
+  affected pixels (those whose winning sample touches the node, and the pixel
+  the node carries) are re-evaluated by the oracle with the node moved by +-h,
+  and their loss recomputed in float64. The assertion uses h = 1 mm (tie-free:
+  h and h/2 agree to 1e-4) and requires <= 1e-3 relative error. At h = 5 cm
+  near-field rays move their horizon by up to ~0.3 deg, more than the loss's
+  0.02-deg disk-edge rounding, so those differences are secants across the disk
+  edge; they are printed, not asserted;
 * run-to-run bit-identical gradients (fixed-point scatter), and the
   torch.autograd.Function wrapper.
 """
@@ -147,6 +149,7 @@ def test_adjoint_matches_finite_differences_of_the_float64_oracle():
     rng = np.random.default_rng(1)
     checked = rejected = 0
     worst = 0.0
+    coarse = []
     for dem_index, count in ((0, 10), (1, 6)):
         done = 0
         g = grads[dem_index]
@@ -162,7 +165,7 @@ def test_adjoint_matches_finite_differences_of_the_float64_oracle():
             if not affected or len(affected) > 120:
                 continue
             fd = []
-            for step in (0.05, 0.025):
+            for step in (1e-3, 5e-4, 0.05):
                 values, moves = [], []
                 for sign in (1.0, -1.0):
                     moved = list(dems)
@@ -171,16 +174,20 @@ def test_adjoint_matches_finite_differences_of_the_float64_oracle():
                     moves.append(actual)
                 fd.append((values[0] - values[1]) / (moves[0] - moves[1]) / norm)
             if abs(fd[0] - fd[1]) > 1e-4 * max(abs(fd[0]), 1e-30):
-                rejected += 1  # a sample or a cull switches inside the step
+                rejected += 1  # a winner switches inside the step
                 continue
             rel = abs(g[y, x] - fd[0]) / max(abs(fd[0]), 1e-30)
-            print(f"DEM {dem_index} node ({x},{y}): adjoint {g[y, x]:.6e} fd {fd[0]:.6e} "
-                  f"rel {rel:.2e} ({len(affected)} px)", flush=True)
+            rel_coarse = abs(g[y, x] - fd[2]) / max(abs(fd[2]), 1e-30)
+            coarse.append(rel_coarse)
+            print(f"DEM {dem_index} node ({x},{y}): adjoint {g[y, x]:.6e} fd(1 mm) {fd[0]:.6e} "
+                  f"rel {rel:.2e}; fd(5 cm) {fd[2]:.6e} rel {rel_coarse:.2e} ({len(affected)} px)",
+                  flush=True)
             worst = max(worst, rel)
             checked += 1
             done += 1
             assert rel <= 1e-3, (dem_index, x, y)
-    print(f"checked {checked} tie-free nodes, rejected {rejected}, worst rel {worst:.2e}", flush=True)
+    print(f"checked {checked} tie-free nodes, rejected {rejected}, worst rel {worst:.2e}; "
+          f"at h = 5 cm within 1e-3: {np.mean(np.array(coarse) <= 1e-3):.2f}", flush=True)
     assert checked >= 8
 
 
@@ -202,7 +209,7 @@ def test_autograd_function_matches_the_problem():
     loss = HorizonSunLoss.apply(problem, *z)
     loss.backward()
     reference = SunLossProblem(dems, [(128, 128)], sun, azimuth_count=N)
-    assert abs(reference.forward() - float(loss)) <= 1e-12
+    assert abs(reference.forward() - loss.item()) <= 1e-12
     want = reference.backward()
     for tensor, w in zip(z, want):
         np.testing.assert_allclose(tensor.grad.numpy(), w.astype(np.float32), rtol=1e-6, atol=1e-30)
