@@ -178,10 +178,7 @@ int lsh_session_destroy(lsh_session* session) {
   hipSetDevice(s->device);
   hipStreamSynchronize(s->stream);
   for (Dem& d : s->dems) {
-    if (d.live) {
-      hipFree(d.level0);
-      hipFree(d.mips);
-    }
+    if (d.live) hipFree(d.level0);  // the mips share the allocation
   }
   if (s->segment_scratch) hipFree(s->segment_scratch);
   if (s->tile_scratch) hipFree(s->tile_scratch);
@@ -318,15 +315,15 @@ int lsh_dem_create(lsh_session* session, const float* elevation, int32_t width, 
   }
   if (int r = select_device(s)) return r;
   std::lock_guard<std::mutex> lock(s->mutex);
+  // level 0 and its mips in one allocation, so the march addresses every level
+  // with 32-bit offsets from one base (buffer loads)
   size_t level0_bytes = sizeof(float) * static_cast<size_t>(width) * height;
   size_t mip_bytes = sizeof(float) * static_cast<size_t>(d.mip_count > 0 ? d.mip_count : 1);
-  LSH_CHECK_HIP(hipMalloc(&d.level0, level0_bytes), "hipMalloc DEM");
-  hipError_t e = hipMalloc(&d.mips, mip_bytes);
-  if (e != hipSuccess) {
-    hipFree(d.level0);
-    set_error(std::string("hipMalloc pyramid: ") + hipGetErrorString(e));
-    return LSH_ENOMEM;
-  }
+  LSH_REQUIRE(level0_bytes + mip_bytes < (static_cast<size_t>(1) << 32),
+              "DEM and pyramid must fit in 4 GiB");
+  LSH_CHECK_HIP(hipMalloc(&d.level0, level0_bytes + mip_bytes), "hipMalloc DEM");
+  d.mips = d.level0 + static_cast<size_t>(width) * height;
+  hipError_t e = hipSuccess;
   e = hipMemcpyAsync(d.level0, elevation, level0_bytes, hipMemcpyHostToDevice, s->stream);
   if (e == hipSuccess && mips_host && d.mip_count > 0)
     e = hipMemcpyAsync(d.mips, mips_host, sizeof(float) * d.mip_count, hipMemcpyHostToDevice,
@@ -345,7 +342,6 @@ int lsh_dem_create(lsh_session* session, const float* elevation, int32_t width, 
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   if (e != hipSuccess) {
     hipFree(d.level0);
-    hipFree(d.mips);
     set_error(std::string("DEM upload / pyramid build: ") + hipGetErrorString(e));
     return LSH_EDEVICE;
   }
@@ -361,8 +357,7 @@ int lsh_dem_destroy(lsh_session* session, int32_t dem_id) {
   LSH_REQUIRE(d, "unknown DEM id");
   if (int r = select_device(s)) return r;
   hipStreamSynchronize(s->stream);
-  hipFree(d->level0);
-  hipFree(d->mips);
+  hipFree(d->level0);  // the mips share the allocation
   d->live = false;
   return LSH_OK;
 }

@@ -46,8 +46,8 @@ struct Dem {
   int32_t width = 0, height = 0;
   double geo_transform[6] = {0};
   double projection[6] = {0};
-  float* level0 = nullptr;  // device float32[h][w]
-  float* mips = nullptr;    // device float32[mip_count] (levels 1..L-1)
+  float* level0 = nullptr;  // device float32[h][w], followed in the same allocation by
+  float* mips = nullptr;    // float32[mip_count] (levels 1..L-1) = level0 + w * h
   int64_t mip_count = 0;
   int32_t level_count = 0;
   int32_t levels[kMaxLevels][4] = {{0}};  // (level, offset, width, height)
@@ -56,14 +56,15 @@ struct Dem {
 
 // Per-pass description of a DEM as the march kernel reads it.
 struct MarchDem {
-  const float* level0;
+  const float* level0;      // level 0, followed by the mips (one allocation)
   const float* mips;
+  uint32_t buffer_bytes;    // bytes of level 0 + mips
   int32_t width, height, level_count;
   float map_resolution;     // (column + row resolution) / 2 (cuda_backend.py:1124-1127)
   float column_resolution;  // hypot(t1, t4) (cuda_backend.py:1032-1035)
   float radius;             // projection radius (cuda_backend.py:1132)
   int32_t all_valid;
-  int32_t level_offset[kMaxLevels];
+  int32_t level_offset[kMaxLevels];  // element offsets from level0 (level 0: 0)
   int32_t level_width[kMaxLevels];
   int32_t level_height[kMaxLevels];
 };

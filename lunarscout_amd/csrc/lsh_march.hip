@@ -79,8 +79,8 @@ __device__ __attribute__((always_inline)) void march_pixel(
         p + sel.i00 * seg_stride, p + sel.i10 * seg_stride, p + sel.i01 * seg_stride,
         p + sel.i11 * seg_stride, sel, column, row,
         primary.column_resolution / d.column_resolution);
-    MarchView view{d.level0, d.mips, s_off[pass], s_w[pass], s_h[pass], d.width, d.height,
-                   d.level_count, d.map_resolution, d.radius, d.all_valid};
+    MarchView view{d.level0, d.buffer_bytes, s_off[pass], s_w[pass], s_h[pass], d.width,
+                   d.height, d.level_count, d.map_resolution, d.radius, d.all_valid};
     lsh_march_pass<double, V, kCap, true>(g, view, oz, pass, current, best, cnt);
   }
   float result = current > V(-1e29) ? static_cast<float>(current) : -INFINITY;
@@ -238,9 +238,11 @@ MarchDem march_dem(const Dem& dem) {
   m.map_resolution = (column_res + row_res) * 0.5f;
   m.radius = static_cast<float>(dem.projection[0]);
   m.all_valid = dem.all_valid;
+  const int64_t level0_floats = static_cast<int64_t>(dem.width) * dem.height;
+  m.buffer_bytes = static_cast<uint32_t>(4 * (level0_floats + (dem.mip_count > 0 ? dem.mip_count : 1)));
   for (int l = 0; l < kMaxLevels; ++l) {
     bool used = l < dem.level_count;
-    m.level_offset[l] = used ? dem.levels[l][1] : 0;
+    m.level_offset[l] = used && l > 0 ? static_cast<int32_t>(level0_floats + dem.levels[l][1]) : 0;
     m.level_width[l] = used ? dem.levels[l][2] : 1;
     m.level_height[l] = used ? dem.levels[l][3] : 1;
   }

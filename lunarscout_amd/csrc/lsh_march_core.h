@@ -43,9 +43,9 @@ namespace lsh {
 constexpr float kBeamWidth = 0.004363323129985824f;  // 2*pi/1440 (cuda_backend.py:699)
 
 struct MarchView {
-  const float* level0;
-  const float* mips;
-  const int32_t* level_offset;
+  const float* base;          // level 0 followed by the mips, one allocation
+  uint32_t buffer_bytes;      // its size (the device loads are bounded by it)
+  const int32_t* level_offset;  // element offset of each level from base
   const int32_t* level_width;
   const int32_t* level_height;
   int32_t width, height, level_count;
@@ -62,6 +62,9 @@ struct PassSegment {
   S sx, sy;
   S a1, a2, a3, a4, b1, b2, b3, b4;
   S s_start, s_end, s_chord, c1, c2, c3;
+  // tangent coefficients 2 a2, 3 a3, 4 a4 (and b), the products tangent()
+  // forms first (cuda_backend.py:73-84)
+  S ta2, ta3, ta4, tb2, tb3, tb4;
 };
 
 struct MarchCounters {
@@ -88,23 +91,29 @@ LSH_HD bool lsh_valid(float v) {
   return !__builtin_isnan(v) && !__builtin_isinf(v) && v > -20000.0f;  // cuda_backend.py:43-45
 }
 
-// evaluate (cuda_backend.py:62-71) on the start offset.
 template <class S>
-LSH_HD S lsh_quartic(S x0, S a1, S a2, S a3, S a4, S d) {
-  S d2 = d * d, d3 = d2 * d, d4 = d2 * d2;
-  S r = x0 + a1 * d;
-  r = r + a2 * d2;
-  r = r + a3 * d3;
-  return r + a4 * d4;
+LSH_HD S lsh_fma(S a, S b, S c) {
+  return sizeof(S) == 4 ? static_cast<S>(__builtin_fmaf(float(a), float(b), float(c)))
+                        : static_cast<S>(__builtin_fma(double(a), double(b), double(c)));
 }
 
-// tangent (cuda_backend.py:73-84)
+// evaluate (cuda_backend.py:62-71) on the start offset, in Horner form with
+// fused multiply-adds: x0 + d (a1 + d (a2 + d (a3 + d a4))).
 template <class S>
-LSH_HD S lsh_tangent(S a1, S a2, S a3, S a4, S d) {
-  S d2 = d * d, d3 = d2 * d;
-  S r = a1 + S(2) * a2 * d;
-  r = r + S(3) * a3 * d2;
-  return r + S(4) * a4 * d3;
+LSH_HD S lsh_quartic(S x0, S a1, S a2, S a3, S a4, S d) {
+  S r = lsh_fma(d, a4, a3);
+  r = lsh_fma(d, r, a2);
+  r = lsh_fma(d, r, a1);
+  return lsh_fma(d, r, x0);
+}
+
+// tangent (cuda_backend.py:73-84) in Horner form: a1 + d (2 a2 + d (3 a3 + d 4 a4)),
+// with t2 = 2 a2, t3 = 3 a3, t4 = 4 a4 formed once per ray.
+template <class S>
+LSH_HD S lsh_tangent(S a1, S t2, S t3, S t4, S d) {
+  S r = lsh_fma(d, t4, t3);
+  r = lsh_fma(d, r, t2);
+  return lsh_fma(d, r, a1);
 }
 
 template <class T>
@@ -152,6 +161,32 @@ LSH_HD bool lsh_bound_cannot_rise(V hd, V distance, bool near, V oz, V radius,
   V rhs = current * current * lxx;
   if (lz <= V(0)) return current >= V(0) || lz * lz >= rhs;
   return current > V(0) && lz * lz <= rhs;
+}
+
+// The 2x2 taps of a bound test or sample, by element offsets from the DEM's
+// base. On the GPU: buffer loads with 32-bit offsets from a descriptor over the
+// DEM's allocation (out-of-range offsets would read 0, never fault).
+LSH_HD void lsh_gather(const MarchView& d, uint32_t o00, uint32_t o10, uint32_t o01, uint32_t o11,
+                       float& v00, float& v10, float& v01, float& v11) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(d.base), static_cast<short>(0), static_cast<int>(d.buffer_bytes),
+      0x00020000);
+  // the b32 loads return the raw 32 bits
+  v00 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                      rsrc, static_cast<int>(o00 * 4u), 0, 0));
+  v10 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                      rsrc, static_cast<int>(o10 * 4u), 0, 0));
+  v01 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                      rsrc, static_cast<int>(o01 * 4u), 0, 0));
+  v11 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                      rsrc, static_cast<int>(o11 * 4u), 0, 0));
+#else
+  v00 = d.base[o00];
+  v10 = d.base[o10];
+  v01 = d.base[o01];
+  v11 = d.base[o11];
+#endif
 }
 
 // Ray distance s in km. Single precision keeps an unevaluated pair (hi + lo)
@@ -231,12 +266,19 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   RayDistance<S> s;
   s.start(lsh_max(g.s_start, S(0.001)));
   int sample_index = 0;
+  // The start level is the number of levels l in 1..top whose cell side 4^l res
+  // fits the beam footprint; the sides grow with l, so it is the largest such l.
+  // It is tracked from position to position (the footprint changes slowly):
+  // `level` with next_side = 4^(level+1) res, both moved by exact factors of 4.
+  int start_level = 0;
+  S next_side = S(4) * res;
   while (s.at_most(g.s_end)) {
     // -- position (cuda_backend.py:672-705)
     const S delta = s.minus(g.s_start);
     const S px = lsh_quartic(g.sx, g.a1, g.a2, g.a3, g.a4, delta);
     const S py = lsh_quartic(g.sy, g.b1, g.b2, g.b3, g.b4, delta);
-    if (px != px || py != py || px < lo_x || py < lo_y || px >= hi_x || py >= hi_y) break;
+    // the reference's px in [0, W-1), py in [0, H-1); NaN fails every comparison
+    if (!(px >= lo_x && py >= lo_y && px < hi_x && py < hi_y)) break;
     const bool near = s.below(S(0.5));
     const S plx = (px - g.sx) * res, ply = (py - g.sy) * res;
     const S planar = lsh_sqrt(plx * plx + ply * ply);
@@ -247,11 +289,17 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     const S true_m = near ? s.hi * S(1000) + s.lo * S(1000) : g.s_chord * S(1000) + chord;
     // start level: the highest level whose cell side fits the beam footprint
     const S footprint = true_m * static_cast<S>(kBeamWidth);
-    int level = 0;
-    for (int l = 1; l <= top_level; ++l)
-      level += static_cast<S>(1 << (l * 2)) * res <= footprint ? 1 : 0;
-    const S dx = lsh_tangent(g.a1, g.a2, g.a3, g.a4, delta);
-    const S dy = lsh_tangent(g.b1, g.b2, g.b3, g.b4, delta);
+    while (start_level < top_level && next_side <= footprint) {
+      ++start_level;
+      next_side = next_side * S(4);
+    }
+    while (start_level > 0 && next_side * S(0.25) > footprint) {
+      --start_level;
+      next_side = next_side * S(0.25);
+    }
+    int level = start_level;
+    const S dx = lsh_tangent(g.a1, g.ta2, g.ta3, g.ta4, delta);
+    const S dy = lsh_tangent(g.b1, g.tb2, g.tb3, g.tb4, delta);
     const S fx = lsh_floor(px), fy = lsh_floor(py);
     const int ix = g.bx + static_cast<int>(fx), iy = g.by + static_cast<int>(fy);
     // -- hierarchy tests at this position (cuda_backend.py:706-868)
@@ -272,15 +320,11 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
       if (kCount) ++cnt.steps;
       // hierarchy_bilinear_bound (cuda_backend.py:47-60): the 2x2 cells clipped to
       // the level (a clipped neighbour repeats the cell, leaving the max unchanged)
-      const float* base = level == 0 ? d.level0 : d.mips + d.level_offset[level];
       const int cx1 = cx + 1 < w ? cx + 1 : w - 1;
       const int cy1 = cy + 1 < h ? cy + 1 : h - 1;
-      const float* r0 = base + cy * w;
-      const float* r1 = base + cy1 * w;
-      v00 = r0[cx];
-      v10 = r0[cx1];
-      v01 = r1[cx];
-      v11 = r1[cx1];
+      const uint32_t r0 = static_cast<uint32_t>(d.level_offset[level] + cy * w);
+      const uint32_t r1 = static_cast<uint32_t>(d.level_offset[level] + cy1 * w);
+      lsh_gather(d, r0 + cx, r0 + cx1, r1 + cx, r1 + cx1, v00, v10, v01, v11);
       if (all_valid) {
         maxh = __builtin_fmaxf(__builtin_fmaxf(v00, v10), __builtin_fmaxf(v01, v11));
       } else {
@@ -469,6 +513,8 @@ LSH_HD PassSegment<S> lsh_interpolate(const float* p00, const float* p10, const 
   g.s_start = LSH_FIELD(12); g.s_end = LSH_FIELD(13); g.s_chord = LSH_FIELD(14);
   g.c1 = LSH_FIELD(15); g.c2 = LSH_FIELD(16); g.c3 = LSH_FIELD(17);
 #undef LSH_FIELD
+  g.ta2 = S(2) * g.a2; g.ta3 = S(3) * g.a3; g.ta4 = S(4) * g.a4;
+  g.tb2 = S(2) * g.b2; g.tb3 = S(3) * g.b3; g.tb4 = S(4) * g.b4;
   return g;
 }
 

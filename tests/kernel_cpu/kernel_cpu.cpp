@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdint>
 #include <limits>
+#include <vector>
 
 #include "../../lunarscout_amd/csrc/lsh_march_core.h"
 
@@ -17,6 +18,7 @@ struct PassTables {
   int32_t off[4][16], lw[4][16], lh[4][16];
   float colres[4], mapres[4];
   int32_t all_valid[4];
+  std::vector<float> combined[4];  // level 0 followed by the mips, as the device holds them
 };
 }  // namespace
 
@@ -61,9 +63,10 @@ float ray(const kc_pyramid* pyrs, int D, const float* segments, int C, int az, i
     lsh::PassSegment<S> g = lsh::lsh_interpolate<S>(
         base + sel.i00 * stride, base + sel.i10 * stride, base + sel.i01 * stride,
         base + sel.i11 * stride, sel, column, row, t.colres[0] / t.colres[p]);
-    lsh::MarchView v{pyrs[p].dem.elevation, pyrs[p].mips, t.off[p], t.lw[p], t.lh[p],
-                     pyrs[p].dem.width, pyrs[p].dem.height, pyrs[p].level_count, t.mapres[p],
-                     float(pyrs[p].dem.projection[0]), t.all_valid[p]};
+    lsh::MarchView v{t.combined[p].data(), uint32_t(4 * t.combined[p].size()), t.off[p],
+                     t.lw[p], t.lh[p], pyrs[p].dem.width, pyrs[p].dem.height,
+                     pyrs[p].level_count, t.mapres[p], float(pyrs[p].dem.projection[0]),
+                     t.all_valid[p]};
     if (cap)
       lsh::lsh_march_pass<S, V, true, true>(g, v, oz, p, current, best, cnt, tr);
     else
@@ -76,9 +79,15 @@ float ray(const kc_pyramid* pyrs, int D, const float* segments, int C, int az, i
 PassTables tables(const kc_pyramid* pyrs, int D) {
   PassTables t;
   for (int p = 0; p < D; ++p) {
+    const int64_t level0 = int64_t(pyrs[p].dem.width) * pyrs[p].dem.height;
+    int64_t mip_count = 0;
+    for (int l = 1; l < pyrs[p].level_count; ++l)
+      mip_count += int64_t(pyrs[p].levels[l * 4 + 2]) * pyrs[p].levels[l * 4 + 3];
+    t.combined[p].assign(pyrs[p].dem.elevation, pyrs[p].dem.elevation + level0);
+    t.combined[p].insert(t.combined[p].end(), pyrs[p].mips, pyrs[p].mips + mip_count);
     for (int l = 0; l < 16; ++l) {
       bool used = l < pyrs[p].level_count;
-      t.off[p][l] = used ? pyrs[p].levels[l * 4 + 1] : 0;
+      t.off[p][l] = used && l > 0 ? int32_t(level0 + pyrs[p].levels[l * 4 + 1]) : 0;
       t.lw[p][l] = used ? pyrs[p].levels[l * 4 + 2] : 1;
       t.lh[p][l] = used ? pyrs[p].levels[l * 4 + 3] : 1;
     }
