@@ -207,10 +207,18 @@ struct RayDistance {
     hi = t + lo;
     lo = lo - (hi - t);
   }
-  LSH_HD bool at_most(S v) const { return hi < v || (hi == v && lo <= S(0)); }
-  LSH_HD bool below(S v) const { return hi < v || (hi == v && lo < S(0)); }
-  LSH_HD S minus(S v) const { return (hi - v) + lo; }
-  LSH_HD S value() const { return hi + lo; }
+  // (in double precision lo stays 0 and the forms reduce to plain hi)
+  LSH_HD bool at_most(S v) const {
+    return sizeof(S) == 8 ? hi <= v : (hi < v || (hi == v && lo <= S(0)));
+  }
+  LSH_HD bool below(S v) const {
+    return sizeof(S) == 8 ? hi < v : (hi < v || (hi == v && lo < S(0)));
+  }
+  LSH_HD S minus(S v) const { return sizeof(S) == 8 ? hi - v : (hi - v) + lo; }
+  LSH_HD S value() const { return sizeof(S) == 8 ? hi : hi + lo; }
+  LSH_HD S times_1000() const {
+    return sizeof(S) == 8 ? hi * S(1000) : hi * S(1000) + lo * S(1000);
+  }
 };
 
 // Distance along the local tangent to the exit of the current cell, in km
@@ -286,7 +294,7 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     S chord = g.c1 * planar;
     chord = chord + g.c2 * p2;
     chord = chord + g.c3 * p3;
-    const S true_m = near ? s.hi * S(1000) + s.lo * S(1000) : g.s_chord * S(1000) + chord;
+    const S true_m = near ? s.times_1000() : g.s_chord * S(1000) + chord;
     // start level: the highest level whose cell side fits the beam footprint
     const S footprint = true_m * static_cast<S>(kBeamWidth);
     while (start_level < top_level && next_side <= footprint) {
@@ -298,6 +306,7 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
       next_side = next_side * S(0.25);
     }
     int level = start_level;
+    S side = next_side * S(0.25);  // 4^level res, exact
     const S dx = lsh_tangent(g.a1, g.ta2, g.ta3, g.ta4, delta);
     const S dy = lsh_tangent(g.b1, g.tb2, g.tb3, g.tb4, delta);
     const S fx = lsh_floor(px), fy = lsh_floor(py);
@@ -342,7 +351,7 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
         if (kCount) ++cnt.tests;
         break;
       }
-      const V true_near = static_cast<V>(lsh_max(true_m - static_cast<S>(scale) * res, S(1)));
+      const V true_near = static_cast<V>(lsh_max(true_m - side, S(1)));
       if (lsh_bound_cannot_rise(static_cast<V>(maxh) - oz, true_near, near, oz, radius,
                                 inv_two_observer_radius, current)) {  // cull
         if (kCount) ++cnt.tests;
@@ -356,11 +365,12 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
       trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx),
                 float(py + g.by), maxh, 0.f, 0.f, 0.f, 0);
       --level;
+      side = side * S(0.25);
     }
     if (ray_out) break;
     // -- cell skip (cull / nodata) and level-0 step, selected per lane
     const S fscale = static_cast<S>(scale);
-    const S fallback = ((fscale * res) * S(0.5)) * S(0.001);
+    const S fallback = (side * S(0.5)) * S(0.001);
     const S exit_distance = lsh_cell_exit(static_cast<S>(cx * scale - g.bx) - px,
                                           static_cast<S>(cy * scale - g.by) - py, fscale, dx, dy);
     const S skip = exit_distance > S(0) ? exit_distance + S(0.0001) : fallback;
