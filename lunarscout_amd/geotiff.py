@@ -43,6 +43,7 @@ _TILE_W, _TILE_H, _TILE_OFFSETS, _TILE_COUNTS = 322, 323, 324, 325
 _SAMPLE_FORMAT = 339
 _PIXEL_SCALE, _TIEPOINT, _TRANSFORMATION = 33550, 33922, 34264
 _GEOKEYS, _GEO_DOUBLES, _GEO_ASCII, _GDAL_NODATA = 34735, 34736, 34737, 42113
+_GDAL_METADATA = 42112
 
 # GeoKeys
 _K_MODEL_TYPE, _K_RASTER_TYPE = 1024, 1025
@@ -84,16 +85,27 @@ def _dtype(bits: int, sample_format: int) -> np.dtype:
     return np.dtype(f"{kind}{bits // 8}")
 
 
-def _read_ifd(data: bytes, endian: str) -> dict[int, tuple]:
-    (offset,) = struct.unpack_from(endian + "I", data, 4)
-    (count,) = struct.unpack_from(endian + "H", data, offset)
+def _read_ifd(data, endian: str, big: bool = False) -> dict[int, tuple]:
+    """Tags of the first IFD of a classic TIFF or a BigTIFF (8-byte offsets)."""
+    if big:
+        (offset,) = struct.unpack_from(endian + "Q", data, 8)
+        (count,) = struct.unpack_from(endian + "Q", data, offset)
+        entry_size, head, inline, entry_fmt, ptr_fmt = 20, 8, 8, "HHQ8s", "Q"
+    else:
+        (offset,) = struct.unpack_from(endian + "I", data, 4)
+        (count,) = struct.unpack_from(endian + "H", data, offset)
+        entry_size, head, inline, entry_fmt, ptr_fmt = 12, 2, 4, "HHI4s", "I"
     tags: dict[int, tuple] = {}
     for i in range(count):
-        tag, typ, n, value = struct.unpack_from(endian + "HHI4s", data, offset + 2 + 12 * i)
+        tag, typ, n, value = struct.unpack_from(endian + entry_fmt, data, offset + head + entry_size * i)
         size = _TYPE_SIZES.get(typ)
         if size is None:
             continue
-        raw = value if size * n <= 4 else data[struct.unpack(endian + "I", value)[0]:][: size * n]
+        if size * n <= inline:
+            raw = bytes(value)
+        else:
+            start = struct.unpack(endian + ptr_fmt, value)[0]
+            raw = bytes(data[start:start + size * n])
         if typ == 2:
             tags[tag] = (raw[:n].split(b"\0")[0].decode("ascii", "replace"),)
         elif typ in (5, 10):
@@ -120,21 +132,24 @@ def _undo_predictor(block: np.ndarray, predictor: int, dtype: np.dtype, endian: 
     return values
 
 
-def read_geotiff(path) -> tuple[np.ndarray, GeoTiffInfo]:
-    """Band 1 of a GeoTIFF as its native dtype, with its georeferencing."""
+def read_geotiff(path, band: int = 1) -> tuple[np.ndarray, GeoTiffInfo]:
+    """One band (1-based) of a GeoTIFF or BigTIFF as its native dtype, with its georeferencing."""
     path = Path(path).expanduser()
     if not path.is_file():
         raise GeoTiffError(f"GeoTIFF file does not exist: {path}", code="geotiff_file_not_found",
                            details={"path": str(path)})
-    data = path.read_bytes()
-    if data[:4] == b"II*\0":
+    with open(path, "rb") as handle:
+        head = handle.read(4)
+    big = head in (b"II+\0", b"MM\0+")
+    if head in (b"II*\0", b"II+\0"):
         endian = "<"
-    elif data[:4] == b"MM\0*":
+    elif head in (b"MM\0*", b"MM\0+"):
         endian = ">"
     else:
         raise GeoTiffError(f"File is not a readable GeoTIFF: {path}",
                            code="geotiff_unreadable_or_unsupported", details={"path": str(path)})
-    tags = _read_ifd(data, endian)
+    data = np.memmap(path, dtype=np.uint8, mode="r")
+    tags = _read_ifd(data, endian, big)
     width, height = int(tags[_WIDTH][0]), int(tags[_HEIGHT][0])
     samples = int(tags.get(_SAMPLES, (1,))[0])
     planar = int(tags.get(_PLANAR, (1,))[0])
@@ -156,19 +171,30 @@ def read_geotiff(path) -> tuple[np.ndarray, GeoTiffInfo]:
         offsets, counts = tags[_STRIP_OFFSETS], tags[_STRIP_COUNTS]
     across = (width + bw - 1) // bw
     down = (height + bh - 1) // bh
+    if not 1 <= band <= samples:
+        raise GeoTiffError("GeoTIFF band is out of range.", code="geotiff_band_out_of_range",
+                           details={"band": band, "band_count": samples})
+    first = (band - 1) * across * down  # band-separate blocks are band-major
     out = np.empty((height, width), dtype=dtype)
-    for index in range(across * down):  # band 1 comes first when band-separate
-        raw = data[offsets[index]: offsets[index] + counts[index]]
+    for index in range(across * down):
+        start, count = offsets[first + index], counts[first + index]
+        by, bx = divmod(index, across)
+        if count == 0:  # a block never written
+            y0, x0 = by * bh, bx * bw
+            out[y0:y0 + bh, x0:x0 + bw] = 0
+            continue
+        raw = bytes(data[start: start + count])
         if compression != 1:
             raw = zlib.decompress(raw)
-        by, bx = divmod(index, across)
         rows = bh if _TILE_OFFSETS in tags else min(bh, height - by * bh)
         block = np.frombuffer(raw, dtype=np.uint8)[: rows * bw * dtype.itemsize]
         values = _undo_predictor(block.reshape(rows, bw * dtype.itemsize), predictor, dtype, endian)
         y0, x0 = by * bh, bx * bw
         h, w = min(rows, height - y0), min(bw, width - x0)
         out[y0:y0 + h, x0:x0 + w] = values[:h, :w]
-    return out, GeoTiffInfo(_affine(tags), _projection(tags), _nodata(tags), width, height)
+    info = GeoTiffInfo(_affine(tags), _projection(tags), _nodata(tags), width, height)
+    del data
+    return out, info
 
 
 def _affine(tags):
@@ -262,23 +288,13 @@ def load_dem(path) -> DemGrid:
 
 # -- writer -----------------------------------------------------------------------------------
 
-def _entry(tag, typ, values, endian, blobs, base):
-    fmt = _TYPE_FORMATS[typ]
-    if typ == 2:
-        raw = values.encode("ascii") + b"\0"
-        n = len(raw)
-    else:
-        n = len(values)
-        raw = struct.pack(endian + fmt * n, *values)
-    if len(raw) <= 4:
-        return struct.pack(endian + "HHI", tag, typ, n) + raw.ljust(4, b"\0")
-    offset = base + sum(len(b) for b in blobs)
-    blobs.append(raw + (b"\0" if len(raw) % 2 else b""))
-    return struct.pack(endian + "HHII", tag, typ, n, offset)
+_TIFF_TYPES = {3: ("H", 2), 4: ("I", 4), 12: ("d", 8), 16: ("Q", 8), 2: ("s", 1)}
 
 
-def _encode_block(values: np.ndarray, predictor: int) -> bytes:
+def _encode_block(values: np.ndarray, predictor: int, compress: bool) -> bytes:
     rows, width = values.shape
+    if not compress:
+        return np.ascontiguousarray(values.astype(values.dtype.newbyteorder("<"))).tobytes()
     if predictor == 3:
         be = np.ascontiguousarray(values.astype(values.dtype.newbyteorder(">")))
         planes = be.view(np.uint8).reshape(rows, width, values.dtype.itemsize).transpose(0, 2, 1)
@@ -290,84 +306,176 @@ def _encode_block(values: np.ndarray, predictor: int) -> bytes:
     return zlib.compress(diff.tobytes(), 6)
 
 
+class GeoTiffWriter:
+    """Streaming writer of a tiled (128x128), band-separate GeoTIFF (geotiff.py:190-260).
+
+    Blocks may be written in any order (``write_block`` / ``write_region``);
+    ``close`` appends the directory and atomically renames the staged file over
+    ``path``. Deflate with predictor 3 (floats) or 2 (integers), as the
+    reference writes its products; a BigTIFF (8-byte offsets) when the raw size
+    could exceed 4 GiB. Blocks never written stay empty (read as zeros).
+    """
+
+    TILE = 128
+
+    def __init__(self, path, width: int, height: int, bands: int, dtype, affine_transform,
+                 projection: ProjectionParameters, *, nodata: float | None = None,
+                 overwrite: bool = False, compress: bool = True,
+                 descriptions: list[str] | None = None):
+        self.path = Path(path).expanduser().resolve()
+        self.dtype = np.dtype(dtype)
+        if self.dtype == np.bool_ or self.dtype.kind not in "uif":
+            raise InputError("unsupported GeoTIFF data type", code="geotiff_data_type_unsupported",
+                             details={"dtype": str(self.dtype)})
+        if self.path.exists() and not overwrite:
+            raise InputError(f"GeoTIFF output already exists: {self.path}", code="output_exists",
+                             details={"path": str(self.path)})
+        self.width, self.height, self.bands = int(width), int(height), int(bands)
+        self.affine = tuple(float(v) for v in affine_transform)
+        self.projection = projection
+        self.nodata = nodata
+        self.compress = bool(compress)
+        self.predictor = (3 if self.dtype.kind == "f" else 2) if self.compress else 1
+        self.across = (self.width + self.TILE - 1) // self.TILE
+        self.down = (self.height + self.TILE - 1) // self.TILE
+        blocks = self.across * self.down * self.bands
+        raw = blocks * self.TILE * self.TILE * self.dtype.itemsize
+        self.big = raw + 16 * blocks + (1 << 20) >= (1 << 32)
+        self.offsets = np.zeros(blocks, dtype=np.uint64)
+        self.counts = np.zeros(blocks, dtype=np.uint64)
+        self.descriptions = descriptions
+        self.path.parent.mkdir(parents=True, exist_ok=True)
+        fd, tmp = tempfile.mkstemp(prefix=f".{self.path.name}.", suffix=".tmp.tif",
+                                   dir=self.path.parent)
+        self.tmp = Path(tmp)
+        self.handle = os.fdopen(fd, "wb")
+        self.handle.write(b"\0" * (16 if self.big else 8))  # header, patched on close
+
+    def write_block(self, band: int, by: int, bx: int, block: np.ndarray) -> None:
+        """One 128x128 block (smaller edge blocks are zero-padded) of a 0-based band."""
+        tile = np.zeros((self.TILE, self.TILE), dtype=self.dtype)
+        part = np.asarray(block, dtype=self.dtype)
+        tile[: part.shape[0], : part.shape[1]] = part
+        payload = _encode_block(tile, self.predictor, self.compress)
+        index = (band * self.down + by) * self.across + bx
+        self.offsets[index] = self.handle.tell()
+        self.counts[index] = len(payload)
+        self.handle.write(payload)
+        if len(payload) % 2:
+            self.handle.write(b"\0")
+
+    def write_region(self, band: int, y0: int, x0: int, values: np.ndarray) -> None:
+        """A block-aligned region (y0, x0 multiples of 128) of a 0-based band."""
+        if y0 % self.TILE or x0 % self.TILE:
+            raise ValueError("regions must start on the 128-pixel block grid")
+        h, w = values.shape
+        for by in range(y0 // self.TILE, (y0 + h + self.TILE - 1) // self.TILE):
+            for bx in range(x0 // self.TILE, (x0 + w + self.TILE - 1) // self.TILE):
+                ry, rx = by * self.TILE - y0, bx * self.TILE - x0
+                self.write_block(band, by, bx, values[ry: ry + self.TILE, rx: rx + self.TILE])
+
+    def _tags(self):
+        t = self.affine
+        geokeys, doubles = _stereographic_geokeys(self.projection)
+        fmt = {"u": 1, "i": 2, "f": 3}[self.dtype.kind]
+        offset_type = 16 if self.big else 4
+        tags = [
+            (_WIDTH, 4, (self.width,)), (_HEIGHT, 4, (self.height,)),
+            (_BITS, 3, (self.dtype.itemsize * 8,) * self.bands),
+            (_COMPRESSION, 3, (8 if self.compress else 1,)), (_PHOTOMETRIC, 3, (1,)),
+            (_SAMPLES, 3, (self.bands,)), (_PLANAR, 3, (2 if self.bands > 1 else 1,)),
+            (_PREDICTOR, 3, (self.predictor,)), (_TILE_W, 3, (self.TILE,)),
+            (_TILE_H, 3, (self.TILE,)),
+            (_TILE_OFFSETS, offset_type, tuple(int(v) for v in self.offsets)),
+            (_TILE_COUNTS, offset_type, tuple(int(v) for v in self.counts)),
+            (_SAMPLE_FORMAT, 3, (fmt,) * self.bands),
+            (_GEOKEYS, 3, geokeys), (_GEO_DOUBLES, 12, doubles),
+        ]
+        if t[2] != 0.0 or t[4] != 0.0:
+            tags.append((_TRANSFORMATION, 12, (t[1], t[2], 0.0, t[0], t[4], t[5], 0.0, t[3],
+                                               0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0)))
+        else:
+            tags.append((_PIXEL_SCALE, 12, (t[1], -t[5], 0.0)))
+            tags.append((_TIEPOINT, 12, (0.0, 0.0, 0.0, t[0], t[3], 0.0)))
+        if self.nodata is not None:
+            tags.append((_GDAL_NODATA, 2, repr(float(self.nodata))))
+        if self.descriptions:
+            items = "".join(f'<Item name="DESCRIPTION" sample="{i}" role="description">{d}</Item>'
+                            for i, d in enumerate(self.descriptions))
+            tags.append((_GDAL_METADATA, 2, f"<GDALMetadata>{items}</GDALMetadata>"))
+        return sorted(tags, key=lambda x: x[0])
+
+    def close(self) -> Path:
+        e = "<"
+        big = self.big
+        tags = self._tags()
+        entry_size, count_fmt, inline = (20, "Q", 8) if big else (12, "H", 4)
+        # out-of-line values first, then the directory
+        values_at = {}
+        for tag, typ, vals in tags:
+            code, size = _TIFF_TYPES[typ]
+            raw = (vals.encode("ascii") + b"\0") if typ == 2 else struct.pack(e + code * len(vals), *vals)
+            if len(raw) > inline:
+                values_at[tag] = self.handle.tell()
+                self.handle.write(raw)
+                if len(raw) % 2:
+                    self.handle.write(b"\0")
+        ifd = self.handle.tell()
+        self.handle.write(struct.pack(e + count_fmt, len(tags)))
+        for tag, typ, vals in tags:
+            code, size = _TIFF_TYPES[typ]
+            raw = (vals.encode("ascii") + b"\0") if typ == 2 else struct.pack(e + code * len(vals), *vals)
+            n = len(raw) if typ == 2 else len(vals)
+            if big:
+                head = struct.pack(e + "HHQ", tag, typ, n)
+                body = raw.ljust(8, b"\0") if len(raw) <= 8 else struct.pack(e + "Q", values_at[tag])
+            else:
+                head = struct.pack(e + "HHI", tag, typ, n)
+                body = raw.ljust(4, b"\0") if len(raw) <= 4 else struct.pack(e + "I", values_at[tag])
+            self.handle.write(head + body)
+        self.handle.write(struct.pack(e + ("Q" if big else "I"), 0))
+        self.handle.seek(0)
+        if big:
+            self.handle.write(b"II+\0" + struct.pack(e + "HHQ", 8, 0, ifd))
+        else:
+            self.handle.write(b"II*\0" + struct.pack(e + "I", ifd))
+        self.handle.flush()
+        os.fsync(self.handle.fileno())
+        self.handle.close()
+        os.replace(self.tmp, self.path)
+        return self.path
+
+    def abort(self) -> None:
+        try:
+            self.handle.close()
+        finally:
+            self.tmp.unlink(missing_ok=True)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        if exc_type is None:
+            self.close()
+        else:
+            self.abort()
+
+
 def write_geotiff(path, array: np.ndarray, affine_transform, projection: ProjectionParameters, *,
                   nodata: float | None = None, overwrite: bool = False) -> Path:
     """Atomically write one band as a tiled 128x128 deflate GeoTIFF (geotiff.py:202-260)."""
-    path = Path(path).expanduser().resolve()
     values = np.asarray(array)
     if values.ndim != 2:
         raise InputError("GeoTIFF output must be a two-dimensional array.",
                          code="geotiff_invalid_array_shape", details={"shape": list(values.shape)})
-    if values.dtype == np.bool_ or values.dtype.kind not in "uif":
-        raise InputError("unsupported GeoTIFF data type", code="geotiff_data_type_unsupported",
-                         details={"dtype": str(values.dtype)})
-    if path.exists() and not overwrite:
-        raise InputError(f"GeoTIFF output already exists: {path}", code="output_exists",
-                         details={"path": str(path)})
-    endian = "<"
-    height, width = values.shape
-    tile = 128
-    predictor = 3 if values.dtype.kind == "f" else 2
-    across, down = (width + tile - 1) // tile, (height + tile - 1) // tile
-    blocks = []
-    for by in range(down):
-        for bx in range(across):
-            block = np.zeros((tile, tile), dtype=values.dtype)
-            part = values[by * tile:(by + 1) * tile, bx * tile:(bx + 1) * tile]
-            block[: part.shape[0], : part.shape[1]] = part
-            blocks.append(_encode_block(block, predictor))
-    t = list(affine_transform)
-    geokeys, doubles = _stereographic_geokeys(projection)
-    fmt_code = {"u": 1, "i": 2, "f": 3}[values.dtype.kind]
-    tags = [
-        (_WIDTH, 4, (width,)), (_HEIGHT, 4, (height,)), (_BITS, 3, (values.dtype.itemsize * 8,)),
-        (_COMPRESSION, 3, (8,)), (_PHOTOMETRIC, 3, (1,)), (_SAMPLES, 3, (1,)), (_PLANAR, 3, (1,)),
-        (_PREDICTOR, 3, (predictor,)), (_TILE_W, 3, (tile,)), (_TILE_H, 3, (tile,)),
-        (_TILE_OFFSETS, 4, None), (_TILE_COUNTS, 4, tuple(len(b) for b in blocks)),
-        (_SAMPLE_FORMAT, 3, (fmt_code,)),
-        (_PIXEL_SCALE, 12, (t[1], -t[5], 0.0)), (_TIEPOINT, 12, (0.0, 0.0, 0.0, t[0], t[3], 0.0)),
-        (_GEOKEYS, 3, geokeys), (_GEO_DOUBLES, 12, doubles),
-    ]
-    if t[2] != 0.0 or t[4] != 0.0:  # rotated or sheared: a full model transformation
-        tags = [x for x in tags if x[0] not in (_PIXEL_SCALE, _TIEPOINT)]
-        tags.append((_TRANSFORMATION, 12, (t[1], t[2], 0.0, t[0], t[4], t[5], 0.0, t[3],
-                                           0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0)))
-    if nodata is not None:
-        tags.append((_GDAL_NODATA, 2, repr(float(nodata))))
-    tags.sort(key=lambda x: x[0])
-    header = 8
-    data_start = header + 2 + 12 * len(tags) + 4
-    # out-of-line tag values follow the IFD, then the compressed blocks; the
-    # values' sizes do not depend on the offsets, so one sizing pass places them
-    sizing: list[bytes] = []
-    for tag, typ, vals in tags:
-        _entry(tag, typ, tuple(0 for _ in blocks) if vals is None else vals, endian, sizing,
-               data_start)
-    position = data_start + sum(len(b) for b in sizing)
-    positions = []
-    for b in blocks:
-        positions.append(position)
-        position += len(b)
-    blobs: list[bytes] = []
-    entries = [_entry(tag, typ, tuple(positions) if vals is None else vals, endian, blobs, data_start)
-               for tag, typ, vals in tags]
-    content = (b"II*\0" + struct.pack("<I", header) + struct.pack("<H", len(tags))
-               + b"".join(entries) + struct.pack("<I", 0) + b"".join(blobs) + b"".join(blocks))
-    path.parent.mkdir(parents=True, exist_ok=True)
-    fd, tmp = tempfile.mkstemp(prefix=f".{path.name}.", suffix=".tmp.tif", dir=path.parent)
+    writer = GeoTiffWriter(path, values.shape[1], values.shape[0], 1, values.dtype,
+                           affine_transform, projection, nodata=nodata, overwrite=overwrite)
     try:
-        with os.fdopen(fd, "wb") as handle:
-            handle.write(content)
-            handle.flush()
-            os.fsync(handle.fileno())
-        os.replace(tmp, path)
+        writer.write_region(0, 0, 0, values)
     except BaseException:
-        try:
-            os.unlink(tmp)
-        except OSError:
-            pass
+        writer.abort()
         raise
-    return path
+    return writer.close()
 
 
 def _stereographic_geokeys(p: ProjectionParameters):

@@ -1,0 +1,99 @@
+"""Lightmap / PSR products over a stored horizon directory on the "cpu" backend.
+
+The products are checked against the calculator they drive (CpuLightingSession,
+itself checked against the C# captures and the float64 oracle in
+test_lighting_cpu.py): every band and every patch lands where the reference
+layout puts it, missing tiles carry invalid_value, and the public errors
+follow the reference's codes.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from lunarscout_amd import geotiff, synthetic, tiles
+from lunarscout_amd.errors import InputError, OperationCancelledError, ProductGenerationError
+from lunarscout_amd.lighting import CpuLightingSession, reduce_sun_vectors_for_psr
+from lunarscout_amd.products import generate_lightmap, generate_psr
+
+
+@pytest.fixture(scope="module")
+def site(tmp_path_factory):
+    root = tmp_path_factory.mktemp("site")
+    dem = synthetic.fbm_crater_dem(256, 5.0, 1)
+    dem = type(dem)(np.ascontiguousarray(dem.elevation_m[:200, :256]), dem.geo_transform, dem.projection)
+    dem_path = geotiff.write_dem(root / "dem.tif", dem)
+    store = tiles.HorizonTileStore(root / "horizons")
+    rng = np.random.default_rng(2)
+    az = np.radians(np.arange(1440) * 0.25)
+    written = {}
+    for ty, tx, vh, vw in ((0, 0, 128, 128), (0, 128, 128, 128), (128, 0, 72, 128)):
+        hz = (rng.uniform(-1.5, 5.0, (vh * vw, 1)) + 1.2 * np.sin(2 * az)[None, :]).astype(np.float32)
+        store.write(ty, tx, 0.0, hz, compress=True, valid_width=vw, valid_height=vh)
+        written[(ty, tx)] = store.read(ty, tx, 0.0)
+    # tile (128, 128) is missing on purpose
+    sun = synthetic.sun_vectors(dem, 40, 6.0, seed=3)
+    return dem, dem_path, root / "horizons", written, sun
+
+
+def test_lightmap_bands_match_the_calculator(site, tmp_path):
+    dem, dem_path, horizons, written, sun = site
+    events = []
+    out = generate_lightmap(dem_path, horizons, tmp_path / "lightmap.tif", sun_vectors_m=sun,
+                            backend="cpu", invalid_value=7, progress_event_callback=events.append)
+    assert events[0].stage == "start" and events[-1].stage == "complete"
+    assert {e.backend for e in events} == {"cpu"}
+    cpu = CpuLightingSession()
+    for band in (0, 17, 39):
+        values, info = geotiff.read_geotiff(out, band=band + 1)
+        assert values.shape == (200, 256) and values.dtype == np.uint8
+        for (ty, tx), hz in written.items():
+            vh, vw = min(128, 200 - ty), min(128, 256 - tx)
+            want = list(cpu.iter_patch_tiles(dem, hz, sun[band:band + 1], tile_y=ty, tile_x=tx,
+                                             valid_height=vh, valid_width=vw))[0]
+            np.testing.assert_array_equal(values[ty:ty + vh, tx:tx + vw], want)
+        assert np.all(values[128:200, 128:256] == 7)
+
+
+def test_psr_matches_the_calculator_and_missing_tiles_are_invalid(site, tmp_path):
+    dem, dem_path, horizons, written, sun = site
+    out = generate_psr(dem_path, horizons, tmp_path / "psr.tif", sun_vectors_m=sun, backend="cpu",
+                       invalid_value=3)
+    mask, _ = geotiff.read_geotiff(out)
+    reduced, _ = reduce_sun_vectors_for_psr(dem, sun)
+    cpu = CpuLightingSession()
+    for (ty, tx), hz in written.items():
+        vh, vw = min(128, 200 - ty), min(128, 256 - tx)
+        np.testing.assert_array_equal(mask[ty:ty + vh, tx:tx + vw],
+                                      cpu.compute_patch(dem, hz, reduced, tile_y=ty, tile_x=tx,
+                                                        valid_height=vh, valid_width=vw))
+    assert np.all(mask[128:200, 128:256] == 3)
+    assert set(np.unique(mask[:128])) <= {0, 255}
+
+
+def test_product_errors(site, tmp_path):
+    dem, dem_path, horizons, written, sun = site
+    cases = [
+        (dict(horizons_path=tmp_path / "nope"), "product_horizons_not_found"),
+        (dict(output_path=tmp_path / "x.png"), "product_output_extension_invalid"),
+        (dict(backend="cuda"), "product_backend_invalid"),
+        (dict(sun_vectors_m=None), "product_vectors_required"),
+        (dict(sun_vectors_m=np.zeros((3, 2))), "product_vectors_invalid"),
+        (dict(invalid_value=300), "product_invalid_value_invalid"),
+    ]
+    for change, code in cases:
+        kwargs = dict(dem_path=dem_path, horizons_path=horizons, output_path=tmp_path / "p.tif",
+                      sun_vectors_m=sun, backend="cpu")
+        kwargs.update(change)
+        with pytest.raises(InputError) as info:
+            generate_psr(**kwargs)
+        assert info.value.code == code, change
+    (tmp_path / "exists.tif").write_bytes(b"x")
+    with pytest.raises(ProductGenerationError) as info:
+        generate_psr(dem_path, horizons, tmp_path / "exists.tif", sun_vectors_m=sun, backend="cpu")
+    assert info.value.code == "product_output_exists"
+    with pytest.raises(OperationCancelledError):
+        generate_lightmap(dem_path, horizons, tmp_path / "c.tif", sun_vectors_m=sun, backend="cpu",
+                          cancellation_requested=lambda: True)
+    assert not (tmp_path / "c.tif").exists() and not list(tmp_path.glob(".c.tif.*"))
