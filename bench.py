@@ -169,6 +169,7 @@ def main() -> int:
         bytes_alg = 16.0 * (S + T) + 256.0 * rays * passes + 4.0 * rays
         achieved = bytes_alg / (march_ms / 1000.0) / 1e9
     traffic = _traffic_from_profile()
+    codec = _codec_leg(horizons)
     long_lighting = None
     if args.year_lighting:
         long_lighting = _year_lighting(session, lighting, primary, horizons, tiles[:per_step])
@@ -204,6 +205,7 @@ def main() -> int:
             "level0_samples_per_tile": S,
             "bound_tests_per_tile": T,
         },
+        "tile_codec": codec,
         "lighting": {
             "value": round(light_px_times / (lighting_ms / 1000.0), 1),
             "unit": "px·times/s (sun + earth + PSR vectors)",
@@ -233,6 +235,26 @@ def main() -> int:
         result["cpu_baseline"] = _cpu_baseline(dems, sun, args.cpu_sample_pixels)
     print(json.dumps(result), flush=True)
     return 0
+
+
+def _codec_leg(horizons):
+    """.cbin encode / decode of one marched full-site tile on the host threads."""
+    from lunarscout_amd import tiles
+
+    degrees = _slot_view(horizons, 0).to_host()
+    payload = tiles.encode_cbin(degrees)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        payload = tiles.encode_cbin(degrees)
+    encode_s = (time.perf_counter() - t0) / 3
+    out = np.empty_like(degrees)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        tiles.decode_cbin(payload, out=out)
+    decode_s = (time.perf_counter() - t0) / 3
+    return {"encode_ms_per_tile": round(encode_s * 1000, 2), "decode_ms_per_tile": round(decode_s * 1000, 2),
+            "compressed_bytes": len(payload), "raw_bytes": int(degrees.nbytes),
+            "threads": len(os.sched_getaffinity(0))}
 
 
 def _year_lighting(session, lighting, primary, horizons, batch):
