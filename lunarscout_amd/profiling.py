@@ -105,6 +105,14 @@ def counter_per_dispatch(directory: Path, kernel: str, counter: str) -> tuple[fl
     return float(np.mean(list(per.values()))), len(per)
 
 
+def march_dispatch_ms(directory: Path) -> list[float]:
+    """Durations (ms) of the march dispatches of a --kernel-trace run, in dispatch order."""
+    rows = [r for r in _rows(_find(directory, "kernel_trace.csv"))
+            if f"{MARCH_KERNEL}<" in r["Kernel_Name"] or f"{MARCH_KERNEL}(" in r["Kernel_Name"]]
+    rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    return [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6 for r in rows]
+
+
 def kernel_stats(directory: Path) -> list[dict]:
     return list(_rows(_find(directory, "kernel_stats.csv")))
 
@@ -124,6 +132,16 @@ def summarize(args) -> int:
         lines.append(f"| `{short}` | {row['Calls']} | {avg:.3f} | {float(row['MinNs']) / 1e6:.3f} | "
                      f"{float(row['MaxNs']) / 1e6:.3f} | {float(row['Percentage']):.2f} |")
     result = {"march_kernel_average_ms": march_avg_ms, "trace_command": args.trace_command}
+    if args.march_window:
+        first, count = (int(v) for v in args.march_window.split(":"))
+        durations = march_dispatch_ms(Path(args.trace))[first:first + count]
+        window_ms = float(np.mean(durations))
+        result.update({"march_kernel_average_ms_all": march_avg_ms,
+                       "march_kernel_average_ms": window_ms,
+                       "march_window": {"first": first, "count": len(durations)}})
+        lines += ["", f"`march_kernel` dispatches {first + 1}-{first + len(durations)} in dispatch order "
+                  f"(the timed tiles of the first workload; {first} warm-up launches precede them): "
+                  f"average {window_ms:.3f} ms, min {min(durations):.3f}, max {max(durations):.3f}."]
     if args.fetch:
         copy_bytes = args.copy_mib * 2**20
         copy_fetch_kib, _ = counter_per_dispatch(Path(args.copy_fetch), COPY_KERNEL, "FETCH_SIZE")
@@ -177,6 +195,7 @@ def main(argv=None) -> int:
     s.add_argument("--copy-write", help="directory of the copy probe's WRITE_SIZE run")
     s.add_argument("--copy-mib", type=int, default=1024)
     s.add_argument("--pmc-command", action="append", default=[])
+    s.add_argument("--march-window", help="FIRST:COUNT march dispatches (dispatch order) to average")
     s.add_argument("--title", default="march_kernel, full site")
     s.add_argument("--note", default="")
     s.add_argument("--out-text", required=True)
