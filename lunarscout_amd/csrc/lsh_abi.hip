@@ -315,14 +315,16 @@ int lsh_dem_create(lsh_session* session, const float* elevation, int32_t width, 
   }
   if (int r = select_device(s)) return r;
   std::lock_guard<std::mutex> lock(s->mutex);
-  // level 0 and its mips in one allocation, so the march addresses every level
-  // with 32-bit offsets from one base (buffer loads)
+  // level 0, its mips and the bound copy of both in one allocation, so the march
+  // addresses every level with 32-bit offsets from one base (buffer loads)
   size_t level0_bytes = sizeof(float) * static_cast<size_t>(width) * height;
   size_t mip_bytes = sizeof(float) * static_cast<size_t>(d.mip_count > 0 ? d.mip_count : 1);
-  LSH_REQUIRE(level0_bytes + mip_bytes < (static_cast<size_t>(1) << 32),
-              "DEM and pyramid must fit in 4 GiB");
-  LSH_CHECK_HIP(hipMalloc(&d.level0, level0_bytes + mip_bytes), "hipMalloc DEM");
+  LSH_REQUIRE(2 * (level0_bytes + mip_bytes) < (static_cast<size_t>(1) << 32),
+              "DEM, pyramid and bounds must fit in 4 GiB");
+  LSH_CHECK_HIP(hipMalloc(&d.level0, 2 * (level0_bytes + mip_bytes)), "hipMalloc DEM");
   d.mips = d.level0 + static_cast<size_t>(width) * height;
+  d.bound_offset = static_cast<int64_t>((level0_bytes + mip_bytes) / sizeof(float));
+  d.bounds = d.level0 + d.bound_offset;
   hipError_t e = hipSuccess;
   e = hipMemcpyAsync(d.level0, elevation, level0_bytes, hipMemcpyHostToDevice, s->stream);
   if (e == hipSuccess && mips_host && d.mip_count > 0)
@@ -339,10 +341,15 @@ int lsh_dem_create(lsh_session* session, const float* elevation, int32_t width, 
       sh = d.levels[l][3];
     }
   }
+  for (int l = 0; l < n && e == hipSuccess; ++l) {
+    const int64_t at = l == 0 ? 0 : static_cast<int64_t>(width) * height + d.levels[l][1];
+    e = lsh::launch_bound_level(d.level0 + at, d.levels[l][2], d.levels[l][3], d.bounds + at,
+                                s->stream);
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   if (e != hipSuccess) {
     hipFree(d.level0);
-    set_error(std::string("DEM upload / pyramid build: ") + hipGetErrorString(e));
+    set_error(std::string("DEM upload / pyramid / bounds build: ") + hipGetErrorString(e));
     return LSH_EDEVICE;
   }
   d.live = true;
@@ -357,7 +364,7 @@ int lsh_dem_destroy(lsh_session* session, int32_t dem_id) {
   LSH_REQUIRE(d, "unknown DEM id");
   if (int r = select_device(s)) return r;
   hipStreamSynchronize(s->stream);
-  hipFree(d->level0);  // the mips share the allocation
+  hipFree(d->level0);  // the mips and bounds share the allocation
   d->live = false;
   return LSH_OK;
 }

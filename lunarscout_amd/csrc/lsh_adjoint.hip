@@ -220,8 +220,8 @@ adjoint_kernel(AdjointArgs a) {
         p + sel.i00 * seg_stride, p + sel.i10 * seg_stride, p + sel.i01 * seg_stride,
         p + sel.i11 * seg_stride, sel, column, row,
         primary.column_resolution / d.column_resolution);
-    MarchView view{d.level0, d.buffer_bytes, s_off[pass], s_w[pass], s_h[pass], d.width,
-                   d.height, d.level_count, d.map_resolution, d.radius, d.all_valid};
+    MarchView view{d.level0, d.buffer_bytes, d.bound_offset, s_off[pass], s_w[pass], s_h[pass],
+                   d.width, d.height, d.level_count, d.map_resolution, d.radius, d.all_valid};
     if (pass < win_pass)
       lsh_march_pass<double, double, false, true>(g, view, oz, pass, current, best, cnt);
     else

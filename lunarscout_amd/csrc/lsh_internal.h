@@ -47,7 +47,9 @@ struct Dem {
   double geo_transform[6] = {0};
   double projection[6] = {0};
   float* level0 = nullptr;  // device float32[h][w], followed in the same allocation by
-  float* mips = nullptr;    // float32[mip_count] (levels 1..L-1) = level0 + w * h
+  float* mips = nullptr;    // float32[mip_count] (levels 1..L-1) = level0 + w * h, then
+  float* bounds = nullptr;  // the 2x2 bound of level 0 and the mips, same layout
+  int64_t bound_offset = 0;  // bounds - level0 (elements)
   int64_t mip_count = 0;
   int32_t level_count = 0;
   int32_t levels[kMaxLevels][4] = {{0}};  // (level, offset, width, height)
@@ -56,9 +58,10 @@ struct Dem {
 
 // Per-pass description of a DEM as the march kernel reads it.
 struct MarchDem {
-  const float* level0;      // level 0, followed by the mips (one allocation)
+  const float* level0;      // level 0, followed by the mips and the bounds (one allocation)
   const float* mips;
-  uint32_t buffer_bytes;    // bytes of level 0 + mips
+  uint32_t buffer_bytes;    // bytes of level 0 + mips + bounds
+  int32_t bound_offset;     // element offset of the bounds from level0
   int32_t width, height, level_count;
   float map_resolution;     // (column + row resolution) / 2 (cuda_backend.py:1124-1127)
   float column_resolution;  // hypot(t1, t4) (cuda_backend.py:1032-1035)
@@ -101,6 +104,7 @@ void sun_disk(double half_angle_deg, float* half_circle, float* max_photons);
 int upload_tiles(Session* s, const lsh_tile* tiles, int32_t count, const lsh_tile** device_tiles);
 
 // Kernel launchers (defined in the .hip files).
+hipError_t launch_bound_level(const float* src, int w, int h, float* dst, hipStream_t stream);
 hipError_t launch_pyramid_level(const float* src, int sw, int sh, float* dst, int dw, int dh,
                                 hipStream_t stream);
 

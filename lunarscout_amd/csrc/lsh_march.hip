@@ -46,8 +46,10 @@ constexpr int kAzimuthsPerGroup = 4;  // wavefronts per workgroup
 
 // One ray fan of one pixel at one azimuth over every DEM pass, written to
 // output[pixel * N + az]. `dems` are the tile's D DEMs (kernel arguments or
-// the batch site table); s_off / s_w / s_h their level tables in LDS.
-template <class V, bool kCap>
+// the batch site table); s_off / s_w / s_h their level tables in LDS. kCount
+// builds the traversal counters (a separate instantiation: counting costs
+// registers and instructions in every iteration).
+template <class V, bool kCap, bool kCount>
 __device__ __attribute__((always_inline)) void march_pixel(
     const MarchDem* dems, const int32_t (*s_off)[kMaxLevels], const int32_t (*s_w)[kMaxLevels],
     const int32_t (*s_h)[kMaxLevels], int D, const float* segments, int N, int C,
@@ -79,9 +81,9 @@ __device__ __attribute__((always_inline)) void march_pixel(
         p + sel.i00 * seg_stride, p + sel.i10 * seg_stride, p + sel.i01 * seg_stride,
         p + sel.i11 * seg_stride, sel, column, row,
         primary.column_resolution / d.column_resolution);
-    MarchView view{d.level0, d.buffer_bytes, s_off[pass], s_w[pass], s_h[pass], d.width,
-                   d.height, d.level_count, d.map_resolution, d.radius, d.all_valid};
-    lsh_march_pass<double, V, kCap, true>(g, view, oz, pass, current, best, cnt);
+    MarchView view{d.level0, d.buffer_bytes, d.bound_offset, s_off[pass], s_w[pass], s_h[pass],
+                   d.width, d.height, d.level_count, d.map_resolution, d.radius, d.all_valid};
+    lsh_march_pass<double, V, kCap, kCount>(g, view, oz, pass, current, best, cnt);
   }
   float result = current > V(-1e29) ? static_cast<float>(current) : -INFINITY;
   if (flags & LSH_HORIZON_DEGREES) {
@@ -90,7 +92,7 @@ __device__ __attribute__((always_inline)) void march_pixel(
   }
   output[out_index] = result;
   if (winners) winners[out_index] = best;
-  if (counters) {
+  if (kCount && counters) {
     // counters[3]: 64 x the wavefront's longest lane (iterations the SIMD issues)
     unsigned long long a = cnt.samples, b = cnt.tests, c = cnt.steps;
     unsigned m = cnt.steps;
@@ -110,7 +112,7 @@ __device__ __attribute__((always_inline)) void march_pixel(
   }
 }
 
-template <class V, bool kCap>
+template <class V, bool kCap, bool kCount>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LSH_MARCH_WAVES_PER_EU)))
 march_kernel(MarchArgs args) {
   __shared__ int32_t s_off[kMaxDems][kMaxLevels];
@@ -141,10 +143,10 @@ march_kernel(MarchArgs args) {
   const int column = (pblock % nbx) * kBlockPixels + lane % kBlockPixels;
   const int row = (pblock / nbx) * kBlockPixels + lane / kBlockPixels;
   if (az >= args.N || column >= args.tile_width || row >= args.tile_height) return;
-  march_pixel<V, kCap>(args.dem, s_off, s_w, s_h, args.D, args.segments, args.N, args.C,
-                       args.tile_column, args.tile_row, args.tile_width, args.subpatch,
-                       args.observer_elevation, args.flags, column, row, az, args.output,
-                       args.winners, args.counters);
+  march_pixel<V, kCap, kCount>(args.dem, s_off, s_w, s_h, args.D, args.segments, args.N,
+                               args.C, args.tile_column, args.tile_row, args.tile_width,
+                               args.subpatch, args.observer_elevation, args.flags, column, row,
+                               az, args.output, args.winners, args.counters);
 }
 
 // The persistent batched march. Work items are (azimuth group, tile, pixel
@@ -206,7 +208,7 @@ march_batch_kernel(BatchMarchArgs a) {
     const int column = (pblock % kNbx) * kBX + lane % kBX;
     const int row = (pblock / kNbx) * kBY + lane / kBX;
     if (az < a.N) {
-      march_pixel<double, false>(ms.dem, s_off, s_w, s_h, ms.D,
+      march_pixel<double, false, false>(ms.dem, s_off, s_w, s_h, ms.D,
                                  a.segments + tile * a.slot_floats, a.N, a.C, t.tile_column,
                                  t.tile_row, 128, a.subpatch, ms.observer_elevation, a.flags,
                                  column, row, az,
@@ -239,7 +241,8 @@ MarchDem march_dem(const Dem& dem) {
   m.radius = static_cast<float>(dem.projection[0]);
   m.all_valid = dem.all_valid;
   const int64_t level0_floats = static_cast<int64_t>(dem.width) * dem.height;
-  m.buffer_bytes = static_cast<uint32_t>(4 * (level0_floats + (dem.mip_count > 0 ? dem.mip_count : 1)));
+  m.bound_offset = static_cast<int32_t>(dem.bound_offset);
+  m.buffer_bytes = static_cast<uint32_t>(8 * dem.bound_offset);
   for (int l = 0; l < kMaxLevels; ++l) {
     bool used = l < dem.level_count;
     m.level_offset[l] = used && l > 0 ? static_cast<int32_t>(level0_floats + dem.levels[l][1]) : 0;
@@ -252,10 +255,15 @@ MarchDem march_dem(const Dem& dem) {
 template <class V>
 void launch_march_values(MarchArgs args, dim3 grid, dim3 block, hipStream_t stream) {
   if (!(args.flags & LSH_HORIZON_COUNTERS)) args.counters = nullptr;
-  if (args.flags & LSH_HORIZON_CAP)
-    hipLaunchKernelGGL((march_kernel<V, true>), grid, block, 0, stream, args);
+  const bool cap = args.flags & LSH_HORIZON_CAP, count = args.counters != nullptr;
+  if (cap && count)
+    hipLaunchKernelGGL((march_kernel<V, true, true>), grid, block, 0, stream, args);
+  else if (cap)
+    hipLaunchKernelGGL((march_kernel<V, true, false>), grid, block, 0, stream, args);
+  else if (count)
+    hipLaunchKernelGGL((march_kernel<V, false, true>), grid, block, 0, stream, args);
   else
-    hipLaunchKernelGGL((march_kernel<V, false>), grid, block, 0, stream, args);
+    hipLaunchKernelGGL((march_kernel<V, false, false>), grid, block, 0, stream, args);
 }
 
 namespace {

@@ -43,8 +43,10 @@ namespace lsh {
 constexpr float kBeamWidth = 0.004363323129985824f;  // 2*pi/1440 (cuda_backend.py:699)
 
 struct MarchView {
-  const float* base;          // level 0 followed by the mips, one allocation
+  // level 0, the mips, then the 2x2 bound of every level (same layout), one allocation
+  const float* base;
   uint32_t buffer_bytes;      // its size (the device loads are bounded by it)
+  int32_t bound_offset;       // element offset of the bound copy from base
   const int32_t* level_offset;  // element offset of each level from base
   const int32_t* level_width;
   const int32_t* level_height;
@@ -126,6 +128,12 @@ LSH_HD T lsh_floor(T x) {
   return sizeof(T) == 4 ? static_cast<T>(__builtin_floorf(static_cast<float>(x)))
                         : static_cast<T>(__builtin_floor(static_cast<double>(x)));
 }
+// x 2^e, exact for the powers of four the schedule scales the resolution by
+template <class T>
+LSH_HD T lsh_ldexp(T x, int e) {
+  return sizeof(T) == 4 ? static_cast<T>(__builtin_ldexpf(static_cast<float>(x), e))
+                        : static_cast<T>(__builtin_ldexp(static_cast<double>(x), e));
+}
 template <class T>
 LSH_HD T lsh_max(T a, T b) { return a > b ? a : (b > a ? b : a); }
 template <class T>
@@ -148,44 +156,38 @@ LSH_HD V lsh_sphere_slope(V hd, V oz, V distance, V radius, V two_observer_radiu
 // possible <= current for the cull test (cuda_backend.py:767-802) without
 // forming the quotient: the bound's slope is lz / lx with lx = sqrt(sq - lz^2)
 // (or 1e-6), and the comparison is decided on lz, lx^2 and current^2. Exact
-// arithmetic gives the same answer; rounding can differ only at ties.
+// arithmetic gives the same answer; rounding can differ only at ties. Every case
+// is evaluated and the answer selected (no branches inside a wavefront).
 template <class V>
-LSH_HD bool lsh_bound_cannot_rise(V hd, V distance, bool near, V oz, V radius,
+LSH_HD bool lsh_bound_cannot_rise(V hd, V distance, bool near, V two_radius_oz,
                                   V inv_two_observer_radius, V current) {
-  if (near) return hd <= current * distance;  // distance >= 1 m
-  V sq = distance * distance;
-  V rs = (V(2) * radius + V(2) * oz) + hd;
-  V lz = (hd * rs - sq) * inv_two_observer_radius;
-  V lxx = sq - lz * lz;
-  if (!(lxx > V(0))) return lz * V(1e6) <= current;
-  V rhs = current * current * lxx;
-  if (lz <= V(0)) return current >= V(0) || lz * lz >= rhs;
-  return current > V(0) && lz * lz <= rhs;
+  const V sq = distance * distance;
+  const V rs = two_radius_oz + hd;  // 2R + h + oz with h = oz + hd
+  const V lz = (hd * rs - sq) * inv_two_observer_radius;
+  const V lxx = sq - lz * lz;
+  const V rhs = current * current * lxx;
+  const V lz2 = lz * lz;
+  const bool degenerate = !(lxx > V(0));
+  const bool below = lz <= V(0);
+  const bool far_result = degenerate ? lz * V(1e6) <= current
+                                     : (below ? (current >= V(0)) | (lz2 >= rhs)
+                                              : (current > V(0)) & (lz2 <= rhs));
+  return near ? hd <= current * distance : far_result;  // near: distance >= 1 m
 }
 
-// The 2x2 taps of a bound test or sample, by element offsets from the DEM's
-// base. On the GPU: buffer loads with 32-bit offsets from a descriptor over the
-// DEM's allocation (out-of-range offsets would read 0, never fault).
-LSH_HD void lsh_gather(const MarchView& d, uint32_t o00, uint32_t o10, uint32_t o01, uint32_t o11,
-                       float& v00, float& v10, float& v01, float& v11) {
+// One element of the DEM's allocation by its offset from the base. On the GPU a
+// buffer load with a 32-bit offset from a descriptor over the allocation
+// (an out-of-range offset would read 0, never fault); the b32 load returns the
+// raw 32 bits.
+LSH_HD float lsh_load(const MarchView& d, uint32_t offset) {
 #if defined(__HIP_DEVICE_COMPILE__)
   const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<float*>(d.base), static_cast<short>(0), static_cast<int>(d.buffer_bytes),
       0x00020000);
-  // the b32 loads return the raw 32 bits
-  v00 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                      rsrc, static_cast<int>(o00 * 4u), 0, 0));
-  v10 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                      rsrc, static_cast<int>(o10 * 4u), 0, 0));
-  v01 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                      rsrc, static_cast<int>(o01 * 4u), 0, 0));
-  v11 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                      rsrc, static_cast<int>(o11 * 4u), 0, 0));
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                       rsrc, static_cast<int>(offset * 4u), 0, 0));
 #else
-  v00 = d.base[o00];
-  v10 = d.base[o10];
-  v01 = d.base[o01];
-  v11 = d.base[o11];
+  return d.base[offset];
 #endif
 }
 
@@ -260,12 +262,11 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   const V two_observer_radius = V(2) * (radius + oz);
   const V inv_two_observer_radius = V(1) / two_observer_radius;
   const V two_radius_oz = V(2) * radius + V(2) * oz;
+  const S chord0 = g.s_chord * S(1000);
   const S min_step = (S(0.5) * res) * S(0.001);
   const S far_step = (S(0.8) * res) * S(0.001);
-  // bounds of the reference's absolute test px in [0, W-1), as offsets
-  const S lo_x = static_cast<S>(-g.bx), lo_y = static_cast<S>(-g.by);
-  const S hi_x = static_cast<S>(d.width - 1 - g.bx);
-  const S hi_y = static_cast<S>(d.height - 1 - g.by);
+  // the reference's px in [0, W-1) is floor(px) in [0, W-2]
+  const S max_x = static_cast<S>(d.width - 2), max_y = static_cast<S>(d.height - 2);
   // the bilinear clamp at width - 1.0001 (cuda_backend.py:113-114), as a fraction
   const S clamp_tx = (static_cast<S>(d.width) - S(1.0001)) - static_cast<S>(d.width - 2);
   const S clamp_ty = (static_cast<S>(d.height) - S(1.0001)) - static_cast<S>(d.height - 2);
@@ -276,17 +277,20 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   int sample_index = 0;
   // The start level is the number of levels l in 1..top whose cell side 4^l res
   // fits the beam footprint; the sides grow with l, so it is the largest such l.
-  // It is tracked from position to position (the footprint changes slowly):
-  // `level` with next_side = 4^(level+1) res, both moved by exact factors of 4.
+  // It is tracked from position to position (the footprint changes slowly); the
+  // sides 4^l res are formed exactly as res 2^(2l).
   int start_level = 0;
-  S next_side = S(4) * res;
   while (s.at_most(g.s_end)) {
     // -- position (cuda_backend.py:672-705)
     const S delta = s.minus(g.s_start);
     const S px = lsh_quartic(g.sx, g.a1, g.a2, g.a3, g.a4, delta);
     const S py = lsh_quartic(g.sy, g.b1, g.b2, g.b3, g.b4, delta);
-    // the reference's px in [0, W-1), py in [0, H-1); NaN fails every comparison
-    if (!(px >= lo_x && py >= lo_y && px < hi_x && py < hi_y)) break;
+    // the reference's px in [0, W-1), py in [0, H-1), tested on the absolute cell
+    // index floor(px) + bx (exact: integers); NaN fails every comparison
+    const S fx = lsh_floor(px), fy = lsh_floor(py);
+    const S ax = fx + static_cast<S>(g.bx), ay = fy + static_cast<S>(g.by);
+    if (!(ax >= S(0) && ay >= S(0) && ax <= max_x && ay <= max_y)) break;
+    const int ix = static_cast<int>(ax), iy = static_cast<int>(ay);
     const bool near = s.below(S(0.5));
     const S plx = (px - g.sx) * res, ply = (py - g.sy) * res;
     const S planar = lsh_sqrt(plx * plx + ply * ply);
@@ -294,89 +298,67 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     S chord = g.c1 * planar;
     chord = chord + g.c2 * p2;
     chord = chord + g.c3 * p3;
-    const S true_m = near ? s.times_1000() : g.s_chord * S(1000) + chord;
+    const S true_m = near ? s.times_1000() : chord0 + chord;
     // start level: the highest level whose cell side fits the beam footprint
     const S footprint = true_m * static_cast<S>(kBeamWidth);
-    while (start_level < top_level && next_side <= footprint) {
+    while (start_level < top_level && lsh_ldexp(res, 2 * start_level + 2) <= footprint)
       ++start_level;
-      next_side = next_side * S(4);
-    }
-    while (start_level > 0 && next_side * S(0.25) > footprint) {
-      --start_level;
-      next_side = next_side * S(0.25);
-    }
+    while (start_level > 0 && lsh_ldexp(res, 2 * start_level) > footprint) --start_level;
     int level = start_level;
-    S side = next_side * S(0.25);  // 4^level res, exact
     const S dx = lsh_tangent(g.a1, g.ta2, g.ta3, g.ta4, delta);
     const S dy = lsh_tangent(g.b1, g.tb2, g.tb3, g.tb4, delta);
-    const S fx = lsh_floor(px), fy = lsh_floor(py);
-    const int ix = g.bx + static_cast<int>(fx), iy = g.by + static_cast<int>(fy);
     // -- hierarchy tests at this position (cuda_backend.py:706-868)
-    bool ray_out = false, sampled = false;
-    int cx = 0, cy = 0, scale = 1;
-    float v00 = 0.f, v10 = 0.f, v01 = 0.f, v11 = 0.f, maxh = 0.f;
-    bool taps_valid = true;
+    // Each level's outcome -- the ray leaves the level (it ends), a nodata
+    // block, a cull, a level-0 sample, or a descent -- is formed without
+    // branches and the loop has one exit.
+    bool ray_out, nodata, culled;
+    int cx, cy, scale;
+    float maxh;
     while (true) {
       const int shift = level * 2;
       scale = 1 << shift;
       cx = ix >> shift;
       cy = iy >> shift;
       const int w = d.level_width[level], h = d.level_height[level];
-      if (!(cx < w && cy < h)) {  // leaves the level: the ray ends
-        ray_out = true;
-        break;
-      }
-      if (kCount) ++cnt.steps;
-      // hierarchy_bilinear_bound (cuda_backend.py:47-60): the 2x2 cells clipped to
-      // the level (a clipped neighbour repeats the cell, leaving the max unchanged)
-      const int cx1 = cx + 1 < w ? cx + 1 : w - 1;
-      const int cy1 = cy + 1 < h ? cy + 1 : h - 1;
-      const uint32_t r0 = static_cast<uint32_t>(d.level_offset[level] + cy * w);
-      const uint32_t r1 = static_cast<uint32_t>(d.level_offset[level] + cy1 * w);
-      lsh_gather(d, r0 + cx, r0 + cx1, r1 + cx, r1 + cx1, v00, v10, v01, v11);
-      if (all_valid) {
-        maxh = __builtin_fmaxf(__builtin_fmaxf(v00, v10), __builtin_fmaxf(v01, v11));
-      } else {
-        const bool has_x = cx + 1 < w, has_y = cy + 1 < h;
-        const bool ok00 = lsh_valid(v00), ok10 = has_x && lsh_valid(v10);
-        const bool ok01 = has_y && lsh_valid(v01), ok11 = has_x && has_y && lsh_valid(v11);
-        taps_valid = ok00 && ok10 && ok01 && ok11;
-        maxh = -32000.0f;
-        if (ok00 && v00 > maxh) maxh = v00;
-        if (ok10 && v10 > maxh) maxh = v10;
-        if (ok01 && v01 > maxh) maxh = v01;
-        if (ok11 && v11 > maxh) maxh = v11;
-      }
-      if (maxh < -20000.0f) {  // nodata block
-        if (kCount) ++cnt.tests;
-        break;
-      }
+      ray_out = !(cx < w && cy < h);
+      // hierarchy_bilinear_bound (cuda_backend.py:47-60) -- the maximum of the
+      // valid elevations of cells (cx..cx+1, cy..cy+1) clipped to the level,
+      // -32000 when none is valid -- read from the bound copy of the level,
+      // where it is stored per cell (one load instead of four)
+      const uint32_t at = ray_out ? 0u : static_cast<uint32_t>(d.level_offset[level] + cy * w + cx);
+      maxh = lsh_load(d, static_cast<uint32_t>(d.bound_offset) + at);
+      nodata = maxh < -20000.0f;
+      const S side = lsh_ldexp(res, 2 * level);  // 4^level res
       const V true_near = static_cast<V>(lsh_max(true_m - side, S(1)));
-      if (lsh_bound_cannot_rise(static_cast<V>(maxh) - oz, true_near, near, oz, radius,
-                                inv_two_observer_radius, current)) {  // cull
-        if (kCount) ++cnt.tests;
-        break;
+      culled = lsh_bound_cannot_rise(static_cast<V>(maxh) - oz, true_near, near, two_radius_oz,
+                                     inv_two_observer_radius, current);
+      if (kCount && !ray_out) {
+        ++cnt.steps;
+        if (nodata || culled || level != 0) ++cnt.tests;
       }
-      if (level == 0) {
-        sampled = true;
-        break;
-      }
-      if (kCount) ++cnt.tests;
+      if (ray_out || nodata || culled || level == 0) break;
       trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx),
                 float(py + g.by), maxh, 0.f, 0.f, 0.f, 0);
       --level;
-      side = side * S(0.25);
     }
     if (ray_out) break;
+    const bool sampled = !nodata && !culled;  // at level 0
     // -- cell skip (cull / nodata) and level-0 step, selected per lane
     const S fscale = static_cast<S>(scale);
-    const S fallback = (side * S(0.5)) * S(0.001);
+    const S fallback = (lsh_ldexp(res, 2 * level) * S(0.5)) * S(0.001);
     const S exit_distance = lsh_cell_exit(static_cast<S>(cx * scale - g.bx) - px,
                                           static_cast<S>(cy * scale - g.by) - py, fscale, dx, dy);
     const S skip = exit_distance > S(0) ? exit_distance + S(0.0001) : fallback;
     S advance = skip;
     if (sampled) {
-      // bilinear (cuda_backend.py:107-132) of h - oz on the same 2x2 taps
+      // bilinear (cuda_backend.py:107-132) of h - oz on the level-0 taps of the
+      // bound just tested (inside the DEM: ix <= W - 2 and iy <= H - 2 here)
+      const uint32_t r0 = static_cast<uint32_t>(iy * d.width + ix);
+      const uint32_t r1 = r0 + static_cast<uint32_t>(d.width);
+      const float v00 = lsh_load(d, r0), v10 = lsh_load(d, r0 + 1);
+      const float v01 = lsh_load(d, r1), v11 = lsh_load(d, r1 + 1);
+      const bool taps_valid =
+          all_valid || (lsh_valid(v00) && lsh_valid(v10) && lsh_valid(v01) && lsh_valid(v11));
       S sx_frac = px - fx, sy_frac = py - fy;
       if (ix == d.width - 2 && sx_frac > clamp_tx) sx_frac = clamp_tx;
       if (iy == d.height - 2 && sy_frac > clamp_ty) sy_frac = clamp_ty;
@@ -419,7 +401,7 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
                 maxh, float(hd), float(slope), float(advance), 4);
     } else {
       trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx),
-                float(py + g.by), maxh, 0.f, 0.f, float(skip), maxh < -20000.0f ? 2 : 1);
+                float(py + g.by), maxh, 0.f, 0.f, float(skip), nodata ? 2 : 1);
     }
     s.add(advance);
   }

@@ -4,8 +4,7 @@
 // build_max_pyramid (pyramid.py:110-138): each destination cell is the maximum of
 // the valid elevations (finite and above -20000 m) of its 4x4 source block,
 // -32000 when none is valid. Maximum is exact, so the result is bit-identical to
-// the reference. One launch per level; each thread reads its block as four
-// 16-byte rows when the block is interior.
+// the reference. One launch per level.
 
 #include "lsh_internal.h"
 
@@ -36,7 +35,33 @@ __global__ void __launch_bounds__(256) pyramid_kernel(const float* __restrict__ 
   dst[static_cast<int64_t>(r) * dw + c] = m;
 }
 
+// The march's bound of a cell: the maximum of the valid values of cells
+// (x..x+1, y..y+1), neighbours clipped to the level (hierarchy_bilinear_bound,
+// cuda_backend.py:47-60; a clipped neighbour repeats the cell), -32000 when none
+// is valid. Stored per cell so that a bound test is one load.
+__global__ void __launch_bounds__(256) bound_kernel(const float* __restrict__ src, int w, int h,
+                                                    float* __restrict__ dst) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  if (x >= w || y >= h) return;
+  int x1 = x + 1 < w ? x + 1 : w - 1, y1 = y + 1 < h ? y + 1 : h - 1;
+  const float* r0 = src + static_cast<int64_t>(y) * w;
+  const float* r1 = src + static_cast<int64_t>(y1) * w;
+  float taps[4] = {r0[x], r0[x1], r1[x], r1[x1]};
+  float m = -32000.0f;
+  for (float v : taps)
+    if (valid(v) && v > m) m = v;
+  dst[static_cast<int64_t>(y) * w + x] = m;
+}
+
 }  // namespace
+
+hipError_t launch_bound_level(const float* src, int w, int h, float* dst, hipStream_t stream) {
+  dim3 block(256);
+  dim3 grid((w + 255) / 256, h);
+  hipLaunchKernelGGL(bound_kernel, grid, block, 0, stream, src, w, h, dst);
+  return hipGetLastError();
+}
 
 hipError_t launch_pyramid_level(const float* src, int sw, int sh, float* dst, int dw, int dh,
                                 hipStream_t stream) {

@@ -18,8 +18,12 @@ struct PassTables {
   int32_t off[4][16], lw[4][16], lh[4][16];
   float colres[4], mapres[4];
   int32_t all_valid[4];
-  std::vector<float> combined[4];  // level 0 followed by the mips, as the device holds them
+  int32_t bound_offset[4];
+  // level 0, the mips, then the 2x2 bound of every level, as the device holds them
+  std::vector<float> combined[4];
 };
+
+bool valid(float v) { return std::isfinite(v) && v > -20000.0f; }
 }  // namespace
 
 extern "C" {
@@ -63,7 +67,8 @@ float ray(const kc_pyramid* pyrs, int D, const float* segments, int C, int az, i
     lsh::PassSegment<S> g = lsh::lsh_interpolate<S>(
         base + sel.i00 * stride, base + sel.i10 * stride, base + sel.i01 * stride,
         base + sel.i11 * stride, sel, column, row, t.colres[0] / t.colres[p]);
-    lsh::MarchView v{t.combined[p].data(), uint32_t(4 * t.combined[p].size()), t.off[p],
+    lsh::MarchView v{t.combined[p].data(), uint32_t(4 * t.combined[p].size()), t.bound_offset[p],
+                     t.off[p],
                      t.lw[p], t.lh[p], pyrs[p].dem.width, pyrs[p].dem.height,
                      pyrs[p].level_count, t.mapres[p], float(pyrs[p].dem.projection[0]),
                      t.all_valid[p]};
@@ -85,11 +90,31 @@ PassTables tables(const kc_pyramid* pyrs, int D) {
       mip_count += int64_t(pyrs[p].levels[l * 4 + 2]) * pyrs[p].levels[l * 4 + 3];
     t.combined[p].assign(pyrs[p].dem.elevation, pyrs[p].dem.elevation + level0);
     t.combined[p].insert(t.combined[p].end(), pyrs[p].mips, pyrs[p].mips + mip_count);
+    if (mip_count == 0) t.combined[p].push_back(0.0f);  // the device's one-float placeholder
     for (int l = 0; l < 16; ++l) {
       bool used = l < pyrs[p].level_count;
       t.off[p][l] = used && l > 0 ? int32_t(level0 + pyrs[p].levels[l * 4 + 1]) : 0;
       t.lw[p][l] = used ? pyrs[p].levels[l * 4 + 2] : 1;
       t.lh[p][l] = used ? pyrs[p].levels[l * 4 + 3] : 1;
+    }
+    // bound of each cell: max of the valid values of its clipped 2x2 block, else -32000
+    const int64_t n = int64_t(t.combined[p].size());
+    t.bound_offset[p] = int32_t(n);
+    t.combined[p].resize(2 * n, 0.0f);
+    for (int l = 0; l < pyrs[p].level_count; ++l) {
+      const float* src = t.combined[p].data() + t.off[p][l];
+      float* dst = t.combined[p].data() + n + t.off[p][l];
+      const int w = t.lw[p][l], h = t.lh[p][l];
+      for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+          const int x1 = x + 1 < w ? x + 1 : w - 1, y1 = y + 1 < h ? y + 1 : h - 1;
+          const float taps[4] = {src[int64_t(y) * w + x], src[int64_t(y) * w + x1],
+                                 src[int64_t(y1) * w + x], src[int64_t(y1) * w + x1]};
+          float m = -32000.0f;
+          for (float v : taps)
+            if (valid(v) && v > m) m = v;
+          dst[int64_t(y) * w + x] = m;
+        }
     }
     const double* g = pyrs[p].dem.geo_transform;
     float t1 = float(g[1]), t4 = float(g[4]), t2 = float(g[2]), t5 = float(g[5]);
