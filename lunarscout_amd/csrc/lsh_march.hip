@@ -33,15 +33,12 @@ namespace {
 constexpr int kBlockPixels = 8;       // 8x8 pixel block per wavefront
 constexpr int kAzimuthsPerGroup = 4;  // wavefronts per workgroup
 
-// The traversal counters are always accumulated: besides feeding the
-// roofline, their per-branch updates keep the compiler from splitting the flat
-// march loop into nested divergent loops (that restructured form ran 9x slower
-// on gfx950 with identical results). They are stored only when requested.
-// The march is latency-bound on its dependent L2 gathers: capping registers
-// for 4 wavefronts per SIMD (a few spills) measured 1.37x faster than the
-// uncapped 2 wavefronts.
+// The march is latency-bound (dependent gathers, FP64 chains): on the full-site
+// tiles, capping registers for 5 wavefronts per SIMD (96 VGPRs, a few spills
+// outside the hottest path) measured 405 ms per tile against 418 ms at 4 and
+// 448 ms at 6 wavefronts; the uncapped build holds 2.
 #ifndef LSH_MARCH_WAVES_PER_EU
-#define LSH_MARCH_WAVES_PER_EU 4
+#define LSH_MARCH_WAVES_PER_EU 5
 #endif
 
 // One ray fan of one pixel at one azimuth over every DEM pass, written to

@@ -134,6 +134,16 @@ LSH_HD T lsh_ldexp(T x, int e) {
   return sizeof(T) == 4 ? static_cast<T>(__builtin_ldexpf(static_cast<float>(x), e))
                         : static_cast<T>(__builtin_ldexp(static_cast<double>(x), e));
 }
+// the binary exponent e of x = f 2^e, f in [0.5, 1)
+template <class T>
+LSH_HD int lsh_exponent(T x) {
+  int e = 0;
+  if (sizeof(T) == 4)
+    (void)__builtin_frexpf(static_cast<float>(x), &e);
+  else
+    (void)__builtin_frexp(static_cast<double>(x), &e);
+  return e;
+}
 template <class T>
 LSH_HD T lsh_max(T a, T b) { return a > b ? a : (b > a ? b : a); }
 template <class T>
@@ -275,11 +285,7 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   RayDistance<S> s;
   s.start(lsh_max(g.s_start, S(0.001)));
   int sample_index = 0;
-  // The start level is the number of levels l in 1..top whose cell side 4^l res
-  // fits the beam footprint; the sides grow with l, so it is the largest such l.
-  // It is tracked from position to position (the footprint changes slowly); the
-  // sides 4^l res are formed exactly as res 2^(2l).
-  int start_level = 0;
+  const int res_exponent = lsh_exponent(res);
   while (s.at_most(g.s_end)) {
     // -- position (cuda_backend.py:672-705)
     const S delta = s.minus(g.s_start);
@@ -299,12 +305,15 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     chord = chord + g.c2 * p2;
     chord = chord + g.c3 * p3;
     const S true_m = near ? s.times_1000() : chord0 + chord;
-    // start level: the highest level whose cell side fits the beam footprint
+    // start level: the highest l in 1..top whose cell side 4^l res (formed
+    // exactly as res 2^(2l)) fits the beam footprint, else 0. With
+    // footprint = f 2^ef and res = r 2^er (f, r in [0.5, 1)), log2(footprint / res)
+    // lies in (ef - er - 1, ef - er + 1), so that l is floor((ef - er + 1) / 2)
+    // or one less; one exact comparison decides. No loop, no branch.
     const S footprint = true_m * static_cast<S>(kBeamWidth);
-    while (start_level < top_level && lsh_ldexp(res, 2 * start_level + 2) <= footprint)
-      ++start_level;
-    while (start_level > 0 && lsh_ldexp(res, 2 * start_level) > footprint) --start_level;
-    int level = start_level;
+    int level = (lsh_exponent(footprint) - res_exponent + 1) >> 1;
+    if (!(lsh_ldexp(res, 2 * level) <= footprint)) --level;
+    level = footprint >= res ? (level < top_level ? level : top_level) : 0;
     const S dx = lsh_tangent(g.a1, g.ta2, g.ta3, g.ta4, delta);
     const S dy = lsh_tangent(g.b1, g.tb2, g.tb3, g.tb4, delta);
     // -- hierarchy tests at this position (cuda_backend.py:706-868)
