@@ -221,7 +221,8 @@ adjoint_kernel(AdjointArgs a) {
         p + sel.i11 * seg_stride, sel, column, row,
         primary.column_resolution / d.column_resolution);
     MarchView view{d.level0, d.buffer_bytes, d.bound_offset, s_off[pass], s_w[pass], s_h[pass],
-                   d.width, d.height, d.level_count, d.map_resolution, d.radius, d.all_valid};
+                   d.width, d.height, d.level_count, d.map_resolution, d.radius, d.all_valid,
+                   d.resolution, d.resolution_exponent};
     if (pass < win_pass)
       lsh_march_pass<double, double, false, true>(g, view, oz, pass, current, best, cnt);
     else

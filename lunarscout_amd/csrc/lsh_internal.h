@@ -67,6 +67,8 @@ struct MarchDem {
   float column_resolution;  // hypot(t1, t4) (cuda_backend.py:1032-1035)
   float radius;             // projection radius (cuda_backend.py:1132)
   int32_t all_valid;
+  double resolution;        // map_resolution widened
+  int32_t resolution_exponent;  // its binary exponent (frexp)
   int32_t level_offset[kMaxLevels];  // element offsets from level0 (level 0: 0)
   int32_t level_width[kMaxLevels];
   int32_t level_height[kMaxLevels];

@@ -79,7 +79,8 @@ __device__ __attribute__((always_inline)) void march_pixel(
         p + sel.i11 * seg_stride, sel, column, row,
         primary.column_resolution / d.column_resolution);
     MarchView view{d.level0, d.buffer_bytes, d.bound_offset, s_off[pass], s_w[pass], s_h[pass],
-                   d.width, d.height, d.level_count, d.map_resolution, d.radius, d.all_valid};
+                   d.width, d.height, d.level_count, d.map_resolution, d.radius, d.all_valid,
+                   d.resolution, d.resolution_exponent};
     lsh_march_pass<double, V, kCap, kCount>(g, view, oz, pass, current, best, cnt);
   }
   float result = current > V(-1e29) ? static_cast<float>(current) : -INFINITY;
@@ -235,6 +236,8 @@ MarchDem march_dem(const Dem& dem) {
   float row_res = std::sqrt(t2 * t2 + t5 * t5);
   m.column_resolution = column_res;
   m.map_resolution = (column_res + row_res) * 0.5f;
+  m.resolution = static_cast<double>(m.map_resolution);
+  (void)std::frexp(m.resolution, &m.resolution_exponent);
   m.radius = static_cast<float>(dem.projection[0]);
   m.all_valid = dem.all_valid;
   const int64_t level0_floats = static_cast<int64_t>(dem.width) * dem.height;

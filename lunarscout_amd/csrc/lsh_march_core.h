@@ -54,6 +54,10 @@ struct MarchView {
   float map_resolution;  // (column + row resolution) / 2
   float radius;
   int32_t all_valid;     // 1 when every level-0 elevation is valid (no nodata tests needed)
+  // map_resolution widened, and its binary exponent (res = f 2^e, f in
+  // [0.5, 1)): uniform values the loop reads from scalar registers
+  double resolution;
+  int32_t resolution_exponent;
 };
 
 // One pass's interpolated segment with its start relative to (bx, by), in the
@@ -267,7 +271,7 @@ template <class S, class V, bool kCap, bool kCount, class Tracer = NoTrace>
 LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, int pass,
                            V& current, int32_t& best, MarchCounters& cnt,
                            const Tracer& trace = Tracer()) {
-  const S res = static_cast<S>(d.map_resolution);
+  const S res = static_cast<S>(d.resolution);
   const V radius = static_cast<V>(d.radius);
   const V two_observer_radius = V(2) * (radius + oz);
   const V inv_two_observer_radius = V(1) / two_observer_radius;
@@ -285,7 +289,6 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   RayDistance<S> s;
   s.start(lsh_max(g.s_start, S(0.001)));
   int sample_index = 0;
-  const int res_exponent = lsh_exponent(res);
   while (s.at_most(g.s_end)) {
     // -- position (cuda_backend.py:672-705)
     const S delta = s.minus(g.s_start);
@@ -311,7 +314,7 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     // lies in (ef - er - 1, ef - er + 1), so that l is floor((ef - er + 1) / 2)
     // or one less; one exact comparison decides. No loop, no branch.
     const S footprint = true_m * static_cast<S>(kBeamWidth);
-    int level = (lsh_exponent(footprint) - res_exponent + 1) >> 1;
+    int level = (lsh_exponent(footprint) - d.resolution_exponent + 1) >> 1;
     if (!(lsh_ldexp(res, 2 * level) <= footprint)) --level;
     level = footprint >= res ? (level < top_level ? level : top_level) : 0;
     const S dx = lsh_tangent(g.a1, g.ta2, g.ta3, g.ta4, delta);

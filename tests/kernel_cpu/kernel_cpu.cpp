@@ -24,6 +24,12 @@ struct PassTables {
 };
 
 bool valid(float v) { return std::isfinite(v) && v > -20000.0f; }
+
+int res_exponent(float res) {
+  int e = 0;
+  (void)std::frexp(double(res), &e);
+  return e;
+}
 }  // namespace
 
 extern "C" {
@@ -71,7 +77,7 @@ float ray(const kc_pyramid* pyrs, int D, const float* segments, int C, int az, i
                      t.off[p],
                      t.lw[p], t.lh[p], pyrs[p].dem.width, pyrs[p].dem.height,
                      pyrs[p].level_count, t.mapres[p], float(pyrs[p].dem.projection[0]),
-                     t.all_valid[p]};
+                     t.all_valid[p], double(t.mapres[p]), res_exponent(t.mapres[p])};
     if (cap)
       lsh::lsh_march_pass<S, V, true, true>(g, v, oz, p, current, best, cnt, tr);
     else
