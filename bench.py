@@ -170,9 +170,10 @@ def main() -> int:
         achieved = bytes_alg / (march_ms / 1000.0) / 1e9
     traffic = _traffic_from_profile()
     codec = _codec_leg(horizons)
-    long_lighting = None
+    long_lighting = reducers = None
     if args.year_lighting:
         long_lighting = _year_lighting(session, lighting, primary, horizons, tiles[:per_step])
+        reducers = _durations_leg(session, lighting, primary, horizons, tiles[:per_step])
     peaks = None
     if not args.no_probes:
         from lunarscout_amd.profiling import probe_peaks
@@ -212,6 +213,7 @@ def main() -> int:
             "kernel_ms_per_batch": round(lighting_ms, 3),
             "times": {"sun": len(sun), "earth": len(earth), "psr": len(psr_vectors)},
             "year": long_lighting,
+            "year_reducers": reducers,
         },
         "roofline": {
             "kernel": "march_kernel",
@@ -284,6 +286,50 @@ def _year_lighting(session, lighting, primary, horizons, batch):
                         f"{len(psr_vectors)} PSR vectors), 2-hour steps over 365 days",
             "value": round(px_times / (ms / 1000.0), 1), "unit": "px·times/s",
             "kernel_ms": round(ms, 3)}
+
+
+def _durations_leg(session, lighting, primary, horizons, batch):
+    """Downstream reducers fused into the lighting walk over the resident tiles:
+    the longest sunlight (fraction >= 0.5) AND Earth-visible (margin >= 0 deg)
+    duration per calendar-month start interval, and the monthly safe haven (Sun
+    fraction < 0.2 during an Earth outage, margin < 2 deg), from 4380 Sun and
+    4380 Earth vectors (2-hour steps over 365 days, seeds 5 and 6)."""
+    from datetime import datetime, timedelta, timezone
+
+    from lunarscout_amd import durations, synthetic
+    from lunarscout_amd._lib import DeviceBuffer, Event
+
+    sun = synthetic.sun_vectors(primary, 4380, 2.0, seed=5)
+    earth = synthetic.earth_vectors(4380, 2.0, seed=6)
+    bufs = [DeviceBuffer.from_host(session, v) for v in (sun, earth)]
+    start = datetime(2027, 1, 1, tzinfo=timezone.utc)
+    times = [start + timedelta(hours=2 * i) for i in range(4380)]
+    stop = times[-1] + timedelta(hours=2)
+    plans = {
+        "longest_sunlight_earth_duration": durations.longest_duration_plan(
+            times, evaluation_start_utc=start, evaluation_stop_utc=stop,
+            start_intervals=durations.monthly_candidate_intervals(start, stop),
+            sun_threshold=0.5, earth_threshold_deg=0.0),
+        "safe_haven": durations.safe_haven_plan(times, sunlight_threshold=0.2,
+                                                earth_threshold_deg=2.0, time_step_hours=2.0),
+    }
+    result = {"workload": f"{len(batch)} resident full-site tiles x 4380 times (Sun + Earth), "
+                          "2-hour steps over 365 days; 12 monthly bands each"}
+    for name, plan in plans.items():
+        out = lighting.duration_tiles(primary, batch, horizons, plan, bufs[0], bufs[1])  # warm-up
+        e0 = Event(session).record()
+        out = lighting.duration_tiles(primary, batch, horizons, plan, bufs[0], bufs[1], out=out)
+        e1 = Event(session).record()
+        ms = e0.elapsed_ms(e1)
+        values = out.to_host()
+        out.free()
+        px_times = len(batch) * PIXELS * plan.time_count
+        result[name] = {"value": round(px_times / (ms / 1000.0), 1), "unit": "px·times/s",
+                        "ms": round(ms, 3), "bands": plan.bands,
+                        "finite_fraction": round(float(np.isfinite(values).mean()), 4)}
+    for b in bufs:
+        b.free()
+    return result
 
 
 def _adjoint_leg(session):

@@ -277,6 +277,73 @@ int lsh_lighting_tiles_host(const float* elevation, int32_t width, int32_t heigh
                             float* sun_mean, float* earth_margin, float* earth_visible,
                             uint8_t* psr, float* psr_margin);
 
+/* Per-pixel reductions of the lighting time series (the downstream products'
+ * stream reducers), computed in the same walk over the times as the signals,
+ * without materialising the [time][pixel] stream:
+ *   LSH_REDUCE_LONGEST_DURATION replaces reduce_longest_candidate_duration_stream
+ *     (mission_duration.py:178-256) fed by the session's fraction / margin tiles
+ *     as mission_duration_pipeline.py:267-318 feeds it: the condition
+ *     signal >= sun_threshold [AND Earth margin >= earth_threshold_deg], float32
+ *     comparisons; per candidate-start interval k (samples
+ *     [interval_first[k], interval_end[k])) the longest duration in hours (days
+ *     when output_days), summed from sample_hours in time order;
+ *   LSH_REDUCE_SAFE_HAVEN replaces reduce_safe_haven_patch_stream
+ *     (safe_haven.py:79-243): low Sun = fraction < sun_threshold, outage = Earth
+ *     margin < earth_threshold_deg; per month band b (month_of[t] = b, or -1) the
+ *     longest low-Sun run touching an outage in b, times step_hours, NaN where the
+ *     band saw no outage or no Earth-above sample.
+ * The sun signal is LSH_SIGNAL_SUN_FRACTION (the 16-slice disk fraction) or
+ * LSH_SIGNAL_SUN_MARGIN (centre elevation - horizon, degrees); the safe haven
+ * always uses the fraction. Earth margins need the earth vectors (same count). */
+#define LSH_REDUCE_LONGEST_DURATION 0
+#define LSH_REDUCE_SAFE_HAVEN 1
+#define LSH_SIGNAL_SUN_FRACTION 0
+#define LSH_SIGNAL_SUN_MARGIN 1
+
+typedef struct {
+  int32_t reducer;              /* LSH_REDUCE_* */
+  int32_t sun_signal;           /* LSH_SIGNAL_* (longest duration) */
+  double sun_threshold;         /* compared in float32 */
+  int32_t use_earth;            /* longest duration: also require the Earth margin */
+  double earth_threshold_deg;   /* compared in float32 */
+  int32_t time_count;           /* T: sun (and earth) vector count */
+  const double* sample_hours;   /* [T] hours each sample stands for (longest duration) */
+  int32_t interval_count;       /* K candidate-start intervals (longest duration) */
+  const int32_t* interval_first;  /* [K] first sample index inside interval k */
+  const int32_t* interval_end;    /* [K] one past the last sample index inside k */
+  int32_t output_days;          /* longest duration in days instead of hours */
+  const int32_t* month_of;      /* [T] month band of each sample or -1 (safe haven) */
+  int32_t band_count;           /* B month bands (safe haven) */
+  double step_hours;            /* safe haven: hours per sample, applied in float32 */
+} lsh_duration_spec;
+
+/* Bytes of the workspace lsh_duration_tiles(_host) needs for tile_count tiles. */
+int lsh_duration_workspace_bytes(const lsh_duration_spec* spec, int32_t tile_count,
+                                 int64_t* bytes);
+
+/* The reductions on the GPU (no reference C ABI: this replaces the numpy
+ * reducers above and the per-time tiles that fed them). tiles (host), horizons
+ * (device) and the vectors (device, float64 [T][3]; earth nullable when unused)
+ * as lsh_lighting_tiles; the spec's arrays are device pointers; workspace
+ * (device) holds lsh_duration_workspace_bytes and needs no initialisation; out
+ * (device) float32 [band][tile][16384], bands = K or B; pixels outside the DEM
+ * get NaN. */
+int lsh_duration_tiles(lsh_session* session, int32_t dem_id, const lsh_tile* tiles,
+                       int32_t tile_count, const float* horizons, int32_t azimuth_count,
+                       int32_t azimuth_major, const double* sun, const double* earth,
+                       double sun_half_angle_deg, const lsh_duration_spec* spec,
+                       void* workspace, float* out);
+
+/* The same reductions on host threads (the "cpu" backend); every pointer,
+ * including the spec's arrays and the workspace, is host memory. */
+int lsh_duration_tiles_host(const float* elevation, int32_t width, int32_t height,
+                            const double* geo_transform, const double* projection,
+                            const lsh_tile* tiles, int32_t tile_count, const float* horizons,
+                            int32_t azimuth_count, int32_t azimuth_major, const double* sun,
+                            const double* earth, double sun_half_angle_deg,
+                            const lsh_duration_spec* spec, int32_t threads, void* workspace,
+                            float* out);
+
 /* Five-viewpoint, per-azimuth-bin maximum-elevation reduction of Sun vectors
  * for PSR (reduce_sun_vectors_for_psr, psr.py:61-91). Host-only float64.
  * elevation: host float32[height][width]; vectors float64[count][3];

@@ -36,6 +36,11 @@ LIGHT_EARTH_VISIBLE = 16
 LIGHT_PSR = 32
 LIGHT_PSR_MARGIN = 64
 
+REDUCE_LONGEST_DURATION = 0
+REDUCE_SAFE_HAVEN = 1
+SIGNAL_SUN_FRACTION = 0
+SIGNAL_SUN_MARGIN = 1
+
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
 _f32 = ctypes.c_float
@@ -45,6 +50,27 @@ _sz = ctypes.c_size_t
 
 _lib = None
 _lock = threading.Lock()
+
+
+class DurationSpec(ctypes.Structure):
+    """lsh_duration_spec (include/lunarscout_hip.h)."""
+
+    _fields_ = [
+        ("reducer", _i32),
+        ("sun_signal", _i32),
+        ("sun_threshold", _f64),
+        ("use_earth", _i32),
+        ("earth_threshold_deg", _f64),
+        ("time_count", _i32),
+        ("sample_hours", _vp),
+        ("interval_count", _i32),
+        ("interval_first", _vp),
+        ("interval_end", _vp),
+        ("output_days", _i32),
+        ("month_of", _vp),
+        ("band_count", _i32),
+        ("step_hours", _f64),
+    ]
 
 # (name, argtypes) of every exported entry point; tests check the library
 # exports each of them.
@@ -75,6 +101,10 @@ SIGNATURES = {
     "lsh_lighting_tiles_host": [_vp, _i32, _i32, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _vp, _i32,
                                 _vp, _i32, _vp, _i32, _f64, _f64, _i32, _i32, _vp, _vp, _vp, _vp,
                                 _vp, _vp, _vp],
+    "lsh_duration_workspace_bytes": [_vp, _i32, ctypes.POINTER(_i64)],
+    "lsh_duration_tiles": [_vp, _i32, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _f64, _vp, _vp, _vp],
+    "lsh_duration_tiles_host": [_vp, _i32, _i32, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _vp, _vp,
+                                _f64, _vp, _i32, _vp, _vp],
     "lsh_psr_reduce": [_vp, _i32, _i32, _vp, _vp, _vp, _i32, _i32, _vp, ctypes.POINTER(_i32)],
     "lsh_sun_disk": [_f64, _vp, _vp],
     "lsh_cbin_encode": [_vp, _i32, _i32, _vp, _i64, ctypes.POINTER(_i64), _i32],

@@ -94,6 +94,21 @@ int upload_tiles(Session* s, const lsh_tile* tiles, int32_t count, const lsh_til
   return LSH_OK;
 }
 
+int upload_tables(Session* s, const void* host, size_t bytes, void** device) {
+  if (s->table_scratch_bytes < bytes) {
+    LSH_CHECK_HIP(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    if (s->table_scratch) hipFree(s->table_scratch);
+    s->table_scratch = nullptr;
+    s->table_scratch_bytes = 0;
+    LSH_CHECK_HIP(hipMalloc(&s->table_scratch, bytes), "hipMalloc tables");
+    s->table_scratch_bytes = bytes;
+  }
+  LSH_CHECK_HIP(hipMemcpyAsync(s->table_scratch, host, bytes, hipMemcpyHostToDevice, s->stream),
+                "copy tables");
+  *device = s->table_scratch;
+  return LSH_OK;
+}
+
 SegmentDem segment_dem(const Dem& d, double maximum_distance_m) {
   SegmentDem g{};
   g.elevation = d.level0;
@@ -178,10 +193,11 @@ int lsh_session_destroy(lsh_session* session) {
   hipSetDevice(s->device);
   hipStreamSynchronize(s->stream);
   for (Dem& d : s->dems) {
-    if (d.live) hipFree(d.level0);  // the mips share the allocation
+    if (d.live) hipFree(d.level0);  // the mips and bounds share the allocation
   }
   if (s->segment_scratch) hipFree(s->segment_scratch);
   if (s->tile_scratch) hipFree(s->tile_scratch);
+  if (s->table_scratch) hipFree(s->table_scratch);
   hipStreamDestroy(s->stream);
   delete s;
   return LSH_OK;

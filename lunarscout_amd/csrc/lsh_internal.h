@@ -96,6 +96,8 @@ struct Session {
   size_t segment_scratch_bytes = 0;
   lsh_tile* tile_scratch = nullptr;  // device copy of a call's tile list
   int32_t tile_scratch_count = 0;
+  void* table_scratch = nullptr;     // device copy of a call's small tables
+  size_t table_scratch_bytes = 0;
   int cu_count = 0;
 };
 
@@ -104,6 +106,8 @@ void sun_disk(double half_angle_deg, float* half_circle, float* max_photons);
 
 // Copies a host tile list into session scratch (grow-only, stream ordered).
 int upload_tiles(Session* s, const lsh_tile* tiles, int32_t count, const lsh_tile** device_tiles);
+// Copies `bytes` of host tables into session scratch (grow-only, stream ordered).
+int upload_tables(Session* s, const void* host, size_t bytes, void** device);
 
 // Kernel launchers (defined in the .hip files).
 hipError_t launch_bound_level(const float* src, int w, int h, float* dst, hipStream_t stream);

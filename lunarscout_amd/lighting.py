@@ -159,6 +159,59 @@ class HipLightingSession:
                 bufs[name].free()
         return out
 
+    # -- duration reductions -------------------------------------------------------
+    def duration_tiles(self, dem: DemGrid, tiles, horizons: DeviceBuffer, plan, sun, earth=None, *,
+                       azimuth_major: bool = False, out: DeviceBuffer | None = None) -> DeviceBuffer:
+        """Longest-duration / safe-haven bands of many tiles in one launch
+        (``lsh_duration_tiles``): float32 [bands][tile][16384] on the device.
+        ``plan`` is a ``durations.DurationPlan``; sun / earth are host arrays or
+        device buffers of [T][3] Moon-ME positions."""
+        s = self.session
+        dem_id = s.dem_id(dem)
+        tile_arr = _tile_array(tiles)
+        n = len(tile_arr)
+        N = horizons.shape[-2] if azimuth_major else horizons.shape[-1]
+        spec = plan.spec()
+        owned = []
+
+        def device(vec):
+            if vec is None or isinstance(vec, DeviceBuffer):
+                return vec
+            buf = DeviceBuffer.from_host(s, validate_vectors(vec))
+            owned.append(buf)
+            return buf
+
+        sun_buf, earth_buf = device(sun), device(earth if plan.needs_earth else None)
+        need = ctypes.c_int64()
+        check(s.lib.lsh_duration_workspace_bytes(ctypes.byref(spec), n, ctypes.byref(need)),
+              "lsh_duration_workspace_bytes")
+        workspace = DeviceBuffer.empty(s, (max(int(need.value), 8),), np.uint8)
+        owned.append(workspace)
+        shape = (plan.bands, n, PATCH_PIXELS)
+        if out is None or out.shape != shape:
+            out = DeviceBuffer.empty(s, shape, np.float32)
+        try:
+            check(s.lib.lsh_duration_tiles(
+                s.handle, dem_id, tile_arr.ctypes.data, n, horizons.ptr, N, 1 if azimuth_major else 0,
+                sun_buf.ptr, earth_buf.ptr if earth_buf is not None else None,
+                self.sun_half_angle_deg, ctypes.byref(spec), workspace.ptr, out.ptr),
+                "lsh_duration_tiles")
+            s.synchronize()
+        finally:
+            for buf in owned:
+                buf.free()
+        return out
+
+    def duration_patch(self, dem, horizons_deg, plan, sun_vectors_m, earth_vectors_m=None, *,
+                       tile_y, tile_x, valid_height=PATCH_SIZE, valid_width=PATCH_SIZE):
+        """One patch's bands, float32 [valid_height, valid_width] each."""
+        self._check_patch(dem, tile_y, tile_x, valid_height, valid_width)
+        hz = self._upload_patch(_finite_patch(horizons_deg).reshape(PATCH_SIZE, PATCH_SIZE, -1))
+        out = self.duration_tiles(dem, [(tile_x, tile_y)], hz, plan, sun_vectors_m, earth_vectors_m)
+        values = out.to_host().reshape(plan.bands, PATCH_SIZE, PATCH_SIZE)
+        out.free()
+        return tuple(np.ascontiguousarray(b[:valid_height, :valid_width]) for b in values)
+
     # -- per-patch drop-ins -------------------------------------------------------
     def _upload_patch(self, horizons_deg) -> DeviceBuffer:
         horizons = np.ascontiguousarray(horizons_deg, dtype=np.float32)
@@ -334,6 +387,43 @@ class CpuLightingSession:
                                              sun_mean=False, earth_visible=False, psr=False,
                                              sun_fraction=True)["sun_fraction"]
             yield values.reshape(len(batch), PATCH_SIZE, PATCH_SIZE)[:, :valid_height, :valid_width]
+
+    def duration_tiles(self, dem: DemGrid, tiles, horizons: np.ndarray, plan, sun, earth=None, *,
+                       azimuth_major: bool = False) -> np.ndarray:
+        """``HipLightingSession.duration_tiles`` on host threads: float32 [bands][tile][16384]."""
+        tile_arr = _tile_array(tiles)
+        n = len(tile_arr)
+        hz = np.ascontiguousarray(horizons, dtype=np.float32)
+        N = hz.shape[-2] if azimuth_major else hz.shape[-1]
+        if hz.size != n * PATCH_PIXELS * N:
+            raise ValueError("horizons must hold [tile][16384][N] (or [tile][N][16384]) values")
+        sun_v = validate_vectors(sun)
+        earth_v = validate_vectors(earth) if (earth is not None and plan.needs_earth) else None
+        spec = plan.spec()
+        need = ctypes.c_int64()
+        check(self.lib.lsh_duration_workspace_bytes(ctypes.byref(spec), n, ctypes.byref(need)),
+              "lsh_duration_workspace_bytes")
+        workspace = np.empty(max(int(need.value), 8), np.uint8)
+        out = np.empty((plan.bands, n, PATCH_PIXELS), np.float32)
+        elevation = np.ascontiguousarray(dem.elevation_m, dtype=np.float32)
+        gt = np.ascontiguousarray(dem.geo_transform, dtype=np.float64)
+        proj = dem.projection.as_array()
+        check(self.lib.lsh_duration_tiles_host(
+            elevation.ctypes.data, dem.width, dem.height, gt.ctypes.data, proj.ctypes.data,
+            tile_arr.ctypes.data, n, hz.ctypes.data, N, 1 if azimuth_major else 0,
+            sun_v.ctypes.data, None if earth_v is None else earth_v.ctypes.data,
+            self.sun_half_angle_deg, ctypes.byref(spec), self.threads, workspace.ctypes.data,
+            out.ctypes.data), "lsh_duration_tiles_host")
+        return out
+
+    def duration_patch(self, dem, horizons_deg, plan, sun_vectors_m, earth_vectors_m=None, *,
+                       tile_y, tile_x, valid_height=PATCH_SIZE, valid_width=PATCH_SIZE):
+        """One patch's bands, float32 [valid_height, valid_width] each."""
+        HipLightingSession._check_patch(dem, tile_y, tile_x, valid_height, valid_width)
+        out = self.duration_tiles(dem, [(tile_x, tile_y)], _finite_patch(horizons_deg), plan,
+                                  sun_vectors_m, earth_vectors_m)
+        values = out.reshape(plan.bands, PATCH_SIZE, PATCH_SIZE)
+        return tuple(np.ascontiguousarray(b[:valid_height, :valid_width]) for b in values)
 
     iter_patch_fraction_tiles = HipLightingSession.iter_patch_fraction_tiles
     iter_patch_margin_tiles = HipLightingSession.iter_patch_margin_tiles
