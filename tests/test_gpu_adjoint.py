@@ -9,7 +9,8 @@
   affected pixels (those whose winning sample touches the node, and the pixel
   the node carries) are re-evaluated by the oracle with the node moved by +-h,
   and their loss recomputed in float64. The assertion uses h = 1 mm (tie-free:
-  h and h/2 agree to 1e-4) and requires <= 1e-3 relative error. At h = 5 cm
+  h and h/2 agree to 1e-4) and requires <= 1e-3 relative error on at least 30
+  nodes (24 primary, 16 regional candidates). At h = 5 cm
   near-field rays move their horizon by up to ~0.3 deg, more than the loss's
   0.02-deg disk-edge rounding, so those differences are secants across the disk
   edge; they are printed, not asserted;
@@ -150,7 +151,7 @@ def test_adjoint_matches_finite_differences_of_the_float64_oracle():
     checked = rejected = 0
     worst = 0.0
     coarse = []
-    for dem_index, count in ((0, 10), (1, 6)):
+    for dem_index, count in ((0, 24), (1, 16)):
         done = 0
         g = grads[dem_index]
         h_, w_ = g.shape
@@ -158,7 +159,7 @@ def test_adjoint_matches_finite_differences_of_the_float64_oracle():
         interior[1:h_ - 2, 1:w_ - 2] = True
         magnitude = np.abs(g) * interior
         candidates = np.argwhere(magnitude > np.quantile(magnitude[magnitude > 0], 0.5))
-        for y, x in candidates[rng.choice(len(candidates), min(3 * count, len(candidates)), replace=False)]:
+        for y, x in candidates[rng.choice(len(candidates), min(5 * count, len(candidates)), replace=False)]:
             if done >= count:
                 break
             affected = _affected(problem, cells, dem_index, int(x), int(y))
@@ -188,7 +189,7 @@ def test_adjoint_matches_finite_differences_of_the_float64_oracle():
             assert rel <= 1e-3, (dem_index, x, y)
     print(f"checked {checked} tie-free nodes, rejected {rejected}, worst rel {worst:.2e}; "
           f"at h = 5 cm within 1e-3: {np.mean(np.array(coarse) <= 1e-3):.2f}", flush=True)
-    assert checked >= 8
+    assert checked >= 30
 
 
 def test_gradients_are_bit_identical_run_to_run():
