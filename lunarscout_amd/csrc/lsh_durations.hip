@@ -125,6 +125,8 @@ int duration_args(const lsh_tile* tiles, int32_t tile_count, int32_t dem_width,
       LSH_REQUIRE(p.month_of[t] >= -1 && p.month_of[t] < p.band_count,
                   "month_of entries must lie in [-1, band_count)");
   }
+  bool sorted = p.reducer == LSH_REDUCE_SAFE_HAVEN;
+  for (int t = 1; sorted && t < T; ++t) sorted = p.month_of[t] >= p.month_of[t - 1];
   lsh::DurationArgs& d = *a;
   d = lsh::DurationArgs{};
   d.light.dem_width = dem_width;
@@ -148,6 +150,7 @@ int duration_args(const lsh_tile* tiles, int32_t tile_count, int32_t dem_width,
   d.K = p.reducer == LSH_REDUCE_LONGEST_DURATION ? p.interval_count : 0;
   d.output_days = p.output_days ? 1 : 0;
   d.B = p.reducer == LSH_REDUCE_SAFE_HAVEN ? p.band_count : 0;
+  d.months_sorted = sorted ? 1 : 0;
   d.step_hours = static_cast<float>(p.step_hours);
   d.out = out;
   return LSH_OK;

@@ -42,6 +42,7 @@ struct DurationArgs {
   int32_t output_days;
   // safe haven
   const int32_t* month_of;  // [T]
+  int32_t months_sorted;    // month_of is non-decreasing (calendar bands of sorted times)
   int32_t B;
   float step_hours;
   // workspace (per band, per tile, per pixel) and output [band][tile][16384]
@@ -88,19 +89,28 @@ LSH_LHD void close_candidate_run(const DurationArgs& a, int64_t slot, int64_t st
 
 // A low-Sun run [s0, b) of length L ends: months it touched during an outage
 // take L (safe_haven.py:196-215), and their touch marks are cleared.
+LSH_LHD void credit_month(const DurationArgs& a, int64_t slot, int64_t stride, int m, int L) {
+  uint8_t& fl = a.flags[m * stride + slot];
+  if (fl & kTouching) {
+    int32_t& bm = a.best[m * stride + slot];
+    if (L > bm) bm = L;
+    fl = static_cast<uint8_t>(fl & ~kTouching);
+  }
+}
+
 LSH_LHD void close_low_sun_run(const DurationArgs& a, int64_t slot, int64_t stride, int s0, int b,
                                int L) {
+  if (a.months_sorted) {  // the run's months are the range month_of[s0] .. month_of[b - 1]
+    const int m0 = a.month_of[s0] > 0 ? a.month_of[s0] : 0;
+    for (int m = m0; m <= a.month_of[b - 1]; ++m) credit_month(a, slot, stride, m, L);
+    return;
+  }
   int last = -2;
   for (int t = s0; t < b; ++t) {
     const int m = a.month_of[t];
     if (m < 0 || m == last) continue;
     last = m;
-    uint8_t& fl = a.flags[m * stride + slot];
-    if (fl & kTouching) {
-      int32_t& bm = a.best[m * stride + slot];
-      if (L > bm) bm = L;
-      fl = static_cast<uint8_t>(fl & ~kTouching);
-    }
+    credit_month(a, slot, stride, m, L);
   }
 }
 

@@ -204,6 +204,13 @@ def test_fused_host_safe_haven_equals_the_oracle_on_the_same_signals():
     finite = np.isfinite(want)
     assert 0.1 < finite.mean() and np.count_nonzero(want[finite]) > 1000
     np.testing.assert_array_equal(got, want)
+    # a band table that is not in time order (the general path of the reduction)
+    perm = np.array([2, 0, 3, 1], np.int32)
+    shuffled = du.safe_haven_plan(times, sunlight_threshold=0.3, earth_threshold_deg=0.5,
+                                  time_step_hours=2.0, month_bands=du.build_month_bands(times),
+                                  month_index_of=perm[plan.month_of])
+    got = cpu.duration_tiles(dem, [tile], horizons, shuffled, sun, earth)[:, 0]
+    np.testing.assert_array_equal(got, want[np.argsort(perm)])
 
 
 # -- products on the cpu backend -------------------------------------------------
