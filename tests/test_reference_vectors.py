@@ -1,0 +1,103 @@
+"""The oracle (and the host path) against golden vectors written by the
+reference's own pure-numpy functions (tests/golden/make_reference_vectors.py)
+for what no captured fixture pins:
+
+* the segment geometry at polar-stereographic latitude (89.5 S) for a two-DEM
+  site: the reference's Python segment builder (geometry.py:490-573). Bars:
+  start positions, s_start, chord start and c1 within 2 float32 ulps, s_end
+  (the bisected last inside sample) within 1 cm;
+  the fitted paths x(d), y(d) over each segment's span within 1e-4 px and the
+  planar-to-chord polynomial within 1e-5 m. The quartic coefficients
+  themselves are not compared one by one: the least-squares fit of a
+  near-straight path is ill-conditioned in its higher coefficients (different
+  FP64 rounding gives different a3 / a4 with the same path to 1e-5 px);
+* the five-viewpoint PSR Sun-vector reduction (psr.py:61-91) of 4380 vectors:
+  the same indices in the same order, from the oracle and from the product's
+  host reduction (lsh_psr_reduce).
+"""
+
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.golden.make_reference_vectors import AZIMUTHS, TILE, TILE_WIDTH, site
+
+GOLDEN = Path(__file__).resolve().parent / "golden" / "reference_polar.npz"
+
+
+def golden():
+    return np.load(GOLDEN)
+
+
+def _ulps(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    return np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
+
+
+def check_segments(got: np.ndarray, want: np.ndarray) -> dict:
+    """The bars above; returns the measured worst cases."""
+    assert got.shape == want.shape
+    for field in (0, 1, 2, 3, 4, 8, 12, 14, 15):
+        assert _ulps(got[..., field], want[..., field]).max() <= 2, field
+    # s_end comes from the bisection of the last inside sample: within 1 cm
+    assert np.abs(got[..., 13].astype(np.float64) - want[..., 13]).max() <= 1e-5
+    g = got.astype(np.float64).reshape(-1, 18)
+    w = want.astype(np.float64).reshape(-1, 18)
+    span = w[:, 13] - w[:, 12]
+    valid = np.isfinite(span) & (span > 0)
+    d = np.linspace(0.0, 1.0, 101)[None, :] * span[valid, None]
+
+    def path(s, origin, a):  # origin + a1 d + a2 d^2 + a3 d^3 + a4 d^4
+        c = s[:, a:a + 4, None]
+        return s[:, origin, None] + d * (c[:, 0] + d * (c[:, 1] + d * (c[:, 2] + d * c[:, 3])))
+
+    dev = max(np.abs(path(g[valid], 2, 4) - path(w[valid], 2, 4)).max(),
+              np.abs(path(g[valid], 3, 8) - path(w[valid], 3, 8)).max())
+    p = np.linspace(0.0, 1.0, 101)[None, :] * (span[valid, None] * 1000.0)
+
+    def chord(s):
+        return p * (s[:, 15, None] + p * (s[:, 16, None] + p * s[:, 17, None]))
+
+    chord_dev = np.abs(chord(g[valid]) - chord(w[valid])).max()
+    assert dev <= 1e-4, dev
+    assert chord_dev <= 1e-5, chord_dev
+    return {"path_px": float(dev), "chord_m": float(chord_dev), "segments": int(valid.sum())}
+
+
+def test_oracle_segments_match_the_reference_at_polar_geometry():
+    primary, regional, _ = site()
+    got = oracle.segments([primary, regional], tile_column=int(TILE[0]), tile_row=int(TILE[1]),
+                          tile_width=TILE_WIDTH, azimuth_count=AZIMUTHS)
+    stats = check_segments(got, golden()["segments"])
+    print(stats)
+    assert stats["segments"] > 1000
+
+
+def test_psr_reduction_matches_the_reference_on_4380_vectors():
+    from lunarscout_amd.lighting import reduce_sun_vectors_for_psr
+
+    primary, _, sun = site()
+    want = golden()["psr_indices"]
+    _, got = reduce_sun_vectors_for_psr(primary, sun)
+    np.testing.assert_array_equal(got, want)
+    _, got_oracle = oracle.psr_reduce(primary, sun)
+    np.testing.assert_array_equal(np.asarray(got_oracle), want)
+    assert len(want) > 1000
+
+
+@pytest.mark.gpu
+def test_gpu_segments_match_the_reference_at_polar_geometry():
+    from lunarscout_amd.hip_backend import HipSession
+
+    primary, regional, _ = site()
+    session = HipSession(0)
+    got = session.build_subpatch_segments([primary, regional], tile_column=int(TILE[0]),
+                                          tile_row=int(TILE[1]), tile_width=TILE_WIDTH,
+                                          azimuth_count=AZIMUTHS)
+    stats = check_segments(np.asarray(got), golden()["segments"])
+    print(stats, flush=True)
