@@ -18,7 +18,14 @@ from the seeds recorded in the file, with lunarscout_amd.synthetic):
   256^2 @ 120 m seed 22), 32 azimuths, 8-px subpatches, observer 0 m;
 * ``psr_indices``: int64, the five-viewpoint PSR Sun-vector reduction
   (``reduce_sun_vectors_for_psr``, psr.py:61-91) of 4380 synthetic Sun vectors
-  (2-hour steps, seed 23) over the primary DEM.
+  (2-hour steps, seed 23) over the primary DEM;
+* ``sun_fraction``: float32 [3000], ``_sun_fraction_reference`` (lightmap.py:32-73)
+  for 30 random 1440-bin horizon rows x 100 random (azimuth, elevation) pairs
+  (``fraction_cases``, seed 24);
+* ``cbin_payload``: uint8, the ``.cbin`` payload of ``_encode_horizons_python``
+  (file_format.py:39-73) for 16 random 1440-bin rows with jumps, clamped
+  values and half-step quantisation ties (``codec_input``, seed 25): per pixel a
+  little-endian u16 length and the block.
 """
 
 from __future__ import annotations
@@ -51,16 +58,47 @@ def site():
     return primary, regional, sun
 
 
+def fraction_cases():
+    """30 horizon rows [30, 1440] and 3000 (row, azimuth, elevation) cases."""
+    rng = np.random.default_rng(24)
+    az = np.arange(1440) * 0.25
+    rows = np.stack([rng.uniform(-1, 2) + rng.uniform(0.2, 3) * np.sin(np.radians(az) * k +
+                     rng.uniform(0, 6.3)) + rng.normal(0, 0.3, 1440)
+                     for k in rng.integers(1, 9, 30)]).astype(np.float32)
+    which = np.repeat(np.arange(30), 100)
+    azimuth = rng.uniform(0.0, 360.0, 3000)
+    azimuth[:20] = [0.0, 0.1, 0.27, 0.395, 0.4, 359.9, 359.75, 90.0, 180.0, 270.0,
+                    0.125, 0.5, 1.0, 359.0, 0.3950001, 0.39499998, 45.25, 0.0001, 359.9999, 120.37]
+    elevation = rng.uniform(-1.5, 4.5, 3000)
+    return rows, which, azimuth, elevation
+
+
+def codec_input():
+    """float32 degrees [16, 1440] exercising every branch of the encoder."""
+    rng = np.random.default_rng(25)
+    x = np.cumsum(rng.normal(0.0, 0.02, (16, 1440)), axis=1) + rng.uniform(-10, 10, (16, 1))
+    x[:, 100:110] += 30.0                      # two-byte deltas
+    x[0, 200:260] = 75.0                       # clamped above +50
+    x[1, 300:380] = -80.0                      # clamped below -50
+    x[2, 500] = -49.9; x[2, 501] = 49.9        # delta beyond the 15-bit clamp
+    q = 50.0 / 32767.0
+    x[3, :64] = (np.arange(64) - 32 + 0.5) * q  # quantisation ties
+    x[4, :] = np.where(np.arange(1440) % 2, 0.0, 62 * q)  # 1-byte edge 62 / 63
+    return x.astype(np.float32)
+
+
 def _reference_modules():
     package = types.ModuleType("reference_horizon")
     package.__path__ = [str(REFERENCE)]
     sys.modules["reference_horizon"] = package
     return (importlib.import_module("reference_horizon.geometry"),
-            importlib.import_module("reference_horizon.psr"))
+            importlib.import_module("reference_horizon.psr"),
+            importlib.import_module("reference_horizon.lightmap"),
+            importlib.import_module("reference_horizon.file_format"))
 
 
 def main() -> None:
-    geometry, psr = _reference_modules()
+    geometry, psr, lightmap, file_format = _reference_modules()
     primary, regional, sun = site()
 
     def reference_dem(dem):
@@ -78,13 +116,22 @@ def main() -> None:
         azimuth_count=AZIMUTHS, maximum_distance_m=1_000_000.0, observer_elevation_m=0.0,
         subpatch_size=8, grid_convergence=geometry.GridConvergenceInput(0.0, 0.0, 0.0))
     _, indices = psr.reduce_sun_vectors_for_psr(dems[0], sun)
+    rows, which, azimuth, elevation = fraction_cases()
+    fractions = np.array([lightmap._sun_fraction_reference(rows[r], float(a), float(e))
+                          for r, a, e in zip(which, azimuth, elevation)], dtype=np.float32)
+    encoded, lengths = file_format._encode_horizons_python(codec_input())
+    payload = b"".join(int(n).to_bytes(2, "little") + encoded[i, :n].tobytes()
+                       for i, n in enumerate(lengths))
     out = HERE / "reference_polar.npz"
     np.savez_compressed(out, segments=segments.astype(np.float32), psr_indices=indices,
+                        sun_fraction=fractions,
+                        cbin_payload=np.frombuffer(payload, dtype=np.uint8),
                         tile=np.array(TILE, np.int32), tile_width=np.int32(TILE_WIDTH),
                         azimuths=np.int32(AZIMUTHS),
                         seeds=np.array([SEEDS["primary"], SEEDS["regional"], SEEDS["sun"]],
                                        np.int32))
-    print(f"wrote {out}: segments {segments.shape}, {len(indices)} PSR vectors kept")
+    print(f"wrote {out}: segments {segments.shape}, {len(indices)} PSR vectors kept, "
+          f"{len(fractions)} sun fractions, {len(payload)} payload bytes")
 
 
 if __name__ == "__main__":

@@ -13,7 +13,13 @@ for what no captured fixture pins:
   FP64 rounding gives different a3 / a4 with the same path to 1e-5 px);
 * the five-viewpoint PSR Sun-vector reduction (psr.py:61-91) of 4380 vectors:
   the same indices in the same order, from the oracle and from the product's
-  host reduction (lsh_psr_reduce).
+  host reduction (lsh_psr_reduce);
+* the C# sun fraction (lightmap.py:32-73) on 3000 random (horizon, azimuth,
+  elevation) cases, including wrap-around and truncation edges: the oracle's
+  float32 mode bit for bit;
+* the .cbin encoder (file_format.py:39-73) on rows with two-byte deltas, clamped
+  values, the 15-bit delta clamp and quantisation ties: the product codec and
+  the oracle byte for byte, and the decoder back to the quantised values.
 """
 
 from __future__ import annotations
@@ -24,7 +30,8 @@ import numpy as np
 import pytest
 
 import oracle
-from tests.golden.make_reference_vectors import AZIMUTHS, TILE, TILE_WIDTH, site
+from tests.golden.make_reference_vectors import (AZIMUTHS, TILE, TILE_WIDTH, codec_input,
+                                                 fraction_cases, site)
 
 GOLDEN = Path(__file__).resolve().parent / "golden" / "reference_polar.npz"
 
@@ -101,3 +108,25 @@ def test_gpu_segments_match_the_reference_at_polar_geometry():
                                           azimuth_count=AZIMUTHS)
     stats = check_segments(np.asarray(got), golden()["segments"])
     print(stats, flush=True)
+
+
+def test_sun_fraction_matches_the_reference_bit_for_bit():
+    rows, which, azimuth, elevation = fraction_cases()
+    want = golden()["sun_fraction"]
+    got = np.array([oracle.sun_fraction(rows[r], float(a), float(e), float32=True)
+                    for r, a, e in zip(which, azimuth, elevation)], dtype=np.float32)
+    np.testing.assert_array_equal(got, want)
+    assert 0.05 < np.mean(want == 0) < 0.95 and np.mean(want == 1) > 0.01
+
+
+def test_cbin_encoder_matches_the_reference_byte_for_byte():
+    from lunarscout_amd import tiles
+
+    degrees = codec_input()
+    want = golden()["cbin_payload"].tobytes()
+    assert tiles.encode_cbin(degrees) == want
+    assert oracle.cbin_payload(degrees) == want
+    decoded = tiles.decode_cbin(want, pixels=16)
+    np.testing.assert_array_equal(decoded, oracle.cbin_decode(want, pixel_count=16))
+    # where no delta was clamped the decode is the quantised value (within half a step)
+    assert np.abs(decoded[5:, :90] - degrees[5:, :90]).max() <= 0.5 * 50.0 / 32767.0 + 1e-6
