@@ -324,10 +324,11 @@ int lsh_duration_workspace_bytes(const lsh_duration_spec* spec, int32_t tile_cou
 /* The reductions on the GPU (no reference C ABI: this replaces the numpy
  * reducers above and the per-time tiles that fed them). tiles (host), horizons
  * (device) and the vectors (device, float64 [T][3]; earth nullable when unused)
- * as lsh_lighting_tiles; the spec's arrays are device pointers; workspace
- * (device) holds lsh_duration_workspace_bytes and needs no initialisation; out
- * (device) float32 [band][tile][16384], bands = K or B; pixels outside the DEM
- * get NaN. */
+ * as lsh_lighting_tiles; the spec and its arrays are host memory (validated,
+ * then copied to the device with the call); workspace (device) holds
+ * lsh_duration_workspace_bytes and needs no initialisation; out (device)
+ * float32 [band][tile][16384], bands = K or B; pixels outside the DEM get NaN.
+ * The call returns once the reduction has finished. */
 int lsh_duration_tiles(lsh_session* session, int32_t dem_id, const lsh_tile* tiles,
                        int32_t tile_count, const float* horizons, int32_t azimuth_count,
                        int32_t azimuth_major, const double* sun, const double* earth,
