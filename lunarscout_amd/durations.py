@@ -15,6 +15,9 @@ Drop-ins for the reference's landed-mission duration and safe-haven products
   (``lsh_duration_tiles``): the signal of each time is reduced as it is
   computed, with no [time][pixel] stream in memory. A ``DurationPlan`` holds the
   per-time tables both backends read;
+* the body-elevation products ``run_sun_elevation_product`` /
+  ``run_earth_elevation_product`` (elevation_pipeline.py:44-236): one float32
+  band per time of the body centre's terrain-relative elevation;
 * the products ``run_sunlight_duration_product``,
   ``run_sun_elevation_duration_product``,
   ``run_sunlight_earth_elevation_duration_product``,
@@ -292,11 +295,15 @@ def duration_patch(calculator, dem: DemGrid, horizons_deg, plan: DurationPlan, s
 # -- products -------------------------------------------------------------------
 
 
-def _run_product(operation: str, plan: DurationPlan, *, dem, horizon_store, output_path,
-                 sun_vectors_m, earth_vectors_m, observer_elevation_m, nodata, compress, overwrite, backend,
-                 progress_callback, progress_event_callback, cancellation_requested, verbose,
-                 device_id) -> Path:
-    from .lighting import select_lighting_session, validate_vectors
+def _float_product(operation: str, *, dem, horizon_store, output_path, band_count: int,
+                   descriptions, compute, observer_elevation_m, nodata, compress, overwrite,
+                   backend, progress_callback, progress_event_callback, cancellation_requested,
+                   verbose, device_id, validate=None) -> Path:
+    """Float32 multi-band product over the DEM's patches: ``compute(calculator, dem,
+    tile, patch)`` yields the band blocks of one patch (the band order of the
+    file); patches without a horizon tile are nodata. ``validate()`` runs after
+    the argument checks and before any backend is touched."""
+    from .lighting import select_lighting_session
     from .pipeline import enumerate_patches
     from .products import _dem, _Progress
     from .tiles import HorizonTileStore
@@ -314,16 +321,8 @@ def _run_product(operation: str, plan: DurationPlan, *, dem, horizon_store, outp
     if output.exists() and not overwrite:
         raise ProductGenerationError("The product output already exists.",
                                      code="product_output_exists", details={"path": str(output)})
-    try:
-        sun = validate_vectors(sun_vectors_m)
-        earth = None if earth_vectors_m is None else validate_vectors(earth_vectors_m)
-    except ValueError as exc:
-        raise InputError(str(exc), code="product_vectors_invalid") from exc
-    if len(sun) != plan.time_count or (earth is not None and len(earth) != plan.time_count):
-        raise InputError("timestamps and body vectors must align", code="product_vectors_invalid",
-                         details={"times": plan.time_count})
-    if plan.needs_earth and earth is None:
-        raise InputError("Earth vectors are required", code="product_vectors_required")
+    if validate is not None:
+        validate()
     try:
         calculator, selected = select_lighting_session(backend, device_id=device_id)
     except HipBackendError as exc:
@@ -336,10 +335,9 @@ def _run_product(operation: str, plan: DurationPlan, *, dem, horizon_store, outp
 
     writer = None
     try:
-        writer = GeoTiffWriter(output, dem.width, dem.height, plan.bands, np.float32,
+        writer = GeoTiffWriter(output, dem.width, dem.height, band_count, np.float32,
                                dem.geo_transform, dem.projection, nodata=nodata,
-                               overwrite=overwrite, compress=compress,
-                               descriptions=plan.descriptions)
+                               overwrite=overwrite, compress=compress, descriptions=descriptions)
         progress("start", 0, len(patches))
         for k, patch in enumerate(patches):
             if cancellation_requested is not None and cancellation_requested():
@@ -349,15 +347,17 @@ def _run_product(operation: str, plan: DurationPlan, *, dem, horizon_store, outp
             tile = store.read(patch.tile_y, patch.tile_x, observer_elevation_m)
             if tile is None:
                 block = np.full((patch.height, patch.width), nodata, np.float32)
-                for band in range(plan.bands):
+                for band in range(band_count):
                     writer.write_block(band, patch.patch_y, patch.patch_x, block)
                 progress("invalid", k + 1, len(patches), patch.tile_y, patch.tile_x)
                 continue
             tile = np.where(np.isfinite(tile), tile, np.float32(-90.0))
-            bands = calculator.duration_patch(dem, tile, plan, sun, earth, tile_y=patch.tile_y,
-                                              tile_x=patch.tile_x, valid_height=patch.height,
-                                              valid_width=patch.width)
-            for band, block in enumerate(bands):
+            for band, block in enumerate(compute(calculator, dem, tile, patch)):
+                if cancellation_requested is not None and cancellation_requested():
+                    raise OperationCancelledError(f"{operation} generation was cancelled.",
+                                                  code=f"{operation}_cancelled",
+                                                  details={"path": str(output)})
+                block = np.asarray(block, dtype=np.float32)
                 if not np.isnan(nodata):
                     block = np.where(np.isnan(block), np.float32(nodata), block)
                 writer.write_block(band, patch.patch_y, patch.patch_x, block)
@@ -384,6 +384,43 @@ def _run_product(operation: str, plan: DurationPlan, *, dem, horizon_store, outp
     finally:
         if writer is not None:
             writer.abort()
+
+
+def _vectors(vectors_m, count, name):
+    from .lighting import validate_vectors
+
+    if vectors_m is None:
+        return None
+    try:
+        vectors = validate_vectors(vectors_m)
+    except ValueError as exc:
+        raise InputError(str(exc), code="product_vectors_invalid") from exc
+    if len(vectors) != count:
+        raise InputError(f"timestamps and {name} vectors must align",
+                         code="product_vectors_invalid",
+                         details={"times": count, "vectors": len(vectors)})
+    return vectors
+
+
+def _run_product(operation: str, plan: DurationPlan, *, sun_vectors_m, earth_vectors_m,
+                 **kwargs) -> Path:
+    state = {}
+
+    def validate():
+        state["sun"] = _vectors(sun_vectors_m, plan.time_count, "Sun")
+        state["earth"] = _vectors(earth_vectors_m, plan.time_count, "Earth")
+        if state["sun"] is None:
+            raise InputError("Sun vectors are required", code="product_vectors_required")
+        if plan.needs_earth and state["earth"] is None:
+            raise InputError("Earth vectors are required", code="product_vectors_required")
+
+    def compute(calculator, dem, tile, patch):
+        return calculator.duration_patch(dem, tile, plan, state["sun"], state["earth"],
+                                         tile_y=patch.tile_y, tile_x=patch.tile_x,
+                                         valid_height=patch.height, valid_width=patch.width)
+
+    return _float_product(operation, band_count=plan.bands, descriptions=plan.descriptions,
+                          compute=compute, validate=validate, **kwargs)
 
 
 def _duration_product(operation, *, sun_signal, sun_threshold, earth_threshold_deg, dem,
@@ -472,10 +509,56 @@ def run_safe_haven_product(*, dem, horizon_store, output_path, times_utc, sun_ve
                         device_id=device_id)
 
 
+def _body_elevation_product(body: str, vectors_m, *, dem, horizon_store, output_path, times_utc,
+                            observer_elevation_m: float = 0.0, nodata: float = float("nan"),
+                            compress: bool = True, overwrite: bool = False, backend: str = "auto",
+                            progress_callback: Callable | None = None,
+                            progress_event_callback: Callable | None = None,
+                            cancellation_requested: Callable | None = None,
+                            verbose: bool = False, device_id: int = 0) -> Path:
+    """One float32 band per time: the terrain-relative elevation of the body's
+    centre, el - lerp(horizon) (elevation_pipeline.py:44-222)."""
+    times = tuple(times_utc)
+    state = {}
+
+    def validate():
+        state["vectors"] = _vectors(vectors_m, len(times), body)
+        if state["vectors"] is None or not len(times):
+            raise InputError(f"{body} vectors and timestamps are required",
+                             code="product_vectors_required")
+
+    def compute(calculator, dem, tile, patch):
+        return calculator.iter_patch_margin_tiles(dem, tile, state["vectors"],
+                                                  tile_y=patch.tile_y, tile_x=patch.tile_x,
+                                                  valid_height=patch.height,
+                                                  valid_width=patch.width)
+
+    return _float_product(f"{body}_elevation", dem=dem, horizon_store=horizon_store,
+                          output_path=output_path, band_count=len(times),
+                          descriptions=[str(t) for t in times], compute=compute,
+                          validate=validate, observer_elevation_m=observer_elevation_m,
+                          nodata=nodata, compress=compress, overwrite=overwrite, backend=backend,
+                          progress_callback=progress_callback,
+                          progress_event_callback=progress_event_callback,
+                          cancellation_requested=cancellation_requested, verbose=verbose,
+                          device_id=device_id)
+
+
+def run_sun_elevation_product(*, sun_vectors_m, **kwargs) -> Path:
+    """Sun-centre local-horizon elevation per time (elevation_pipeline.py:225-229)."""
+    return _body_elevation_product("sun", sun_vectors_m, **kwargs)
+
+
+def run_earth_elevation_product(*, earth_vectors_m, **kwargs) -> Path:
+    """Earth-centre local-horizon elevation per time (elevation_pipeline.py:232-236)."""
+    return _body_elevation_product("earth", earth_vectors_m, **kwargs)
+
+
 __all__ = [
     "CandidateStartInterval", "DurationPlan", "build_month_bands", "candidate_start_intervals",
     "duration_patch", "fixed_candidate_intervals", "longest_duration_plan", "month_indices_map",
-    "monthly_candidate_intervals", "run_safe_haven_product", "run_sun_elevation_duration_product",
+    "monthly_candidate_intervals", "run_earth_elevation_product", "run_safe_haven_product",
+    "run_sun_elevation_duration_product", "run_sun_elevation_product",
     "run_sun_elevation_earth_elevation_duration_product", "run_sunlight_duration_product",
     "run_sunlight_earth_elevation_duration_product", "safe_haven_plan",
     "validate_evaluation_samples", "weekly_candidate_intervals",

@@ -257,3 +257,27 @@ def test_duration_products_write_the_reduced_bands(tmp_path):
                                   times_utc=times, sun_vectors_m=sun[:-1], earth_vectors_m=earth,
                                   time_step_hours=2.0, backend="cpu")
     assert getattr(err.value, "code", None) == "product_vectors_invalid"
+
+
+def test_body_elevation_products_write_one_margin_band_per_time(tmp_path):
+    from lunarscout_amd.geotiff import read_geotiff
+    from lunarscout_amd.tiles import HorizonTileStore
+
+    T = 12
+    dem, horizons, sun, earth, times = _site(T)
+    store = HorizonTileStore(tmp_path / "h")
+    store.write(0, 128, 0.0, horizons.reshape(16384, 1440), compress=False)
+    out = du.run_earth_elevation_product(dem=dem, horizon_store=store,
+                                         output_path=tmp_path / "earth.tif", times_utc=times,
+                                         earth_vectors_m=earth, backend="cpu")
+    cpu = CpuLightingSession()
+    want = list(cpu.iter_patch_margin_tiles(dem, horizons.reshape(128, 128, 1440), earth,
+                                            tile_y=0, tile_x=128))
+    for t in (0, 5, T - 1):
+        data, info = read_geotiff(out, band=t + 1)
+        np.testing.assert_array_equal(data[0:128, 128:256], want[t])
+        assert np.isnan(data[:, :128]).all()
+    with pytest.raises(Exception) as err:
+        du.run_sun_elevation_product(dem=dem, horizon_store=store, output_path=tmp_path / "s.tif",
+                                     times_utc=times[:-1], sun_vectors_m=sun, backend="cpu")
+    assert getattr(err.value, "code", None) == "product_vectors_invalid"
