@@ -208,6 +208,21 @@ class DurationPlan:
     def needs_earth(self) -> bool:
         return self.reducer == _lib.REDUCE_SAFE_HAVEN or self.use_earth
 
+    def identity(self) -> dict:
+        """JSON identity of the plan (the resumable store's job configuration)."""
+        def digest(a):
+            return None if a is None else _sha256(a)
+
+        return {"reducer": self.reducer, "time_count": self.time_count,
+                "sun_signal": self.sun_signal, "sun_threshold": float(self.sun_threshold),
+                "use_earth": bool(self.use_earth),
+                "earth_threshold_deg": float(self.earth_threshold_deg),
+                "sample_hours": digest(self.sample_hours),
+                "interval_first": digest(self.interval_first),
+                "interval_end": digest(self.interval_end), "output_days": bool(self.output_days),
+                "month_of": digest(self.month_of), "band_count": int(self.band_count),
+                "step_hours": float(self.step_hours)}
+
     def spec(self) -> _lib.DurationSpec:
         """The C struct; its pointers stay valid while this plan lives."""
 
@@ -298,11 +313,14 @@ def duration_patch(calculator, dem: DemGrid, horizons_deg, plan: DurationPlan, s
 def _float_product(operation: str, *, dem, horizon_store, output_path, band_count: int,
                    descriptions, compute, observer_elevation_m, nodata, compress, overwrite,
                    backend, progress_callback, progress_event_callback, cancellation_requested,
-                   verbose, device_id, validate=None) -> Path:
+                   verbose, device_id, validate=None, configuration=None,
+                   start_fresh: bool = False) -> Path:
     """Float32 multi-band product over the DEM's patches: ``compute(calculator, dem,
     tile, patch)`` yields the band blocks of one patch (the band order of the
     file); patches without a horizon tile are nodata. ``validate()`` runs after
-    the argument checks and before any backend is touched."""
+    the argument checks and before any backend is touched. Written through the
+    resumable store: an interrupted run of the same job resumes at the first
+    patch not yet journaled (``start_fresh`` discards a staged product)."""
     from .lighting import select_lighting_session
     from .pipeline import enumerate_patches
     from .products import _dem, _Progress
@@ -323,6 +341,7 @@ def _float_product(operation: str, *, dem, horizon_store, output_path, band_coun
                                      code="product_output_exists", details={"path": str(output)})
     if validate is not None:
         validate()
+    configuration = configuration() if callable(configuration) else (configuration or {})
     try:
         calculator, selected = select_lighting_session(backend, device_id=device_id)
     except HipBackendError as exc:
@@ -331,42 +350,50 @@ def _float_product(operation: str, *, dem, horizon_store, output_path, band_coun
     progress = _Progress(operation, output, selected, verbose, progress_callback,
                          progress_event_callback)
     patches = enumerate_patches(dem.width, dem.height)
-    from .geotiff import GeoTiffWriter
+    from .product_store import ResumableTiledProduct
 
-    writer = None
+    product = None
+
+    def check_cancel():
+        if cancellation_requested is not None and cancellation_requested():
+            raise OperationCancelledError(f"{operation} generation was cancelled.",
+                                          code=f"{operation}_cancelled",
+                                          details={"path": str(output)})
+
+    def bands(tile, patch):
+        for block in compute(calculator, dem, tile, patch):
+            check_cancel()
+            block = np.asarray(block, dtype=np.float32)
+            if not np.isnan(nodata):
+                block = np.where(np.isnan(block), np.float32(nodata), block)
+            yield block
+
     try:
-        writer = GeoTiffWriter(output, dem.width, dem.height, band_count, np.float32,
-                               dem.geo_transform, dem.projection, nodata=nodata,
-                               overwrite=overwrite, compress=compress, descriptions=descriptions)
-        progress("start", 0, len(patches))
+        product = ResumableTiledProduct(
+            output, width=dem.width, height=dem.height, bands=band_count, dtype=np.float32,
+            affine_transform=dem.geo_transform, projection=dem.projection, nodata=nodata,
+            compress=compress, descriptions=descriptions, algorithm=operation,
+            configuration=dict(configuration, observer_elevation_m=float(observer_elevation_m)),
+            overwrite=overwrite, start_fresh=start_fresh)
+        progress("start", len(product.completed), len(patches))
         for k, patch in enumerate(patches):
-            if cancellation_requested is not None and cancellation_requested():
-                raise OperationCancelledError(f"{operation} generation was cancelled.",
-                                              code=f"{operation}_cancelled",
-                                              details={"path": str(output)})
+            if product.is_complete(patch.tile_y, patch.tile_x):
+                continue
+            check_cancel()
             tile = store.read(patch.tile_y, patch.tile_x, observer_elevation_m)
             if tile is None:
-                block = np.full((patch.height, patch.width), nodata, np.float32)
-                for band in range(band_count):
-                    writer.write_block(band, patch.patch_y, patch.patch_x, block)
+                product.write_invalid_patch(patch.tile_y, patch.tile_x, patch.height,
+                                            patch.width, nodata)
                 progress("invalid", k + 1, len(patches), patch.tile_y, patch.tile_x)
                 continue
             tile = np.where(np.isfinite(tile), tile, np.float32(-90.0))
-            for band, block in enumerate(compute(calculator, dem, tile, patch)):
-                if cancellation_requested is not None and cancellation_requested():
-                    raise OperationCancelledError(f"{operation} generation was cancelled.",
-                                                  code=f"{operation}_cancelled",
-                                                  details={"path": str(output)})
-                block = np.asarray(block, dtype=np.float32)
-                if not np.isnan(nodata):
-                    block = np.where(np.isnan(block), np.float32(nodata), block)
-                writer.write_block(band, patch.patch_y, patch.patch_x, block)
+            product.write_patch(patch.tile_y, patch.tile_x, bands(tile, patch))
             progress("valid", k + 1, len(patches), patch.tile_y, patch.tile_x)
-        path = writer.close()
-        writer = None
+        path = product.finalize()
+        product = None
         progress("complete", len(patches), len(patches))
         return path
-    except (OperationCancelledError, InputError):
+    except (OperationCancelledError, InputError, ProductGenerationError):
         raise
     except HipExecutionError as exc:
         raise HipError(f"The HIP {operation} backend failed during execution.",
@@ -382,8 +409,16 @@ def _float_product(operation: str, *, dem, horizon_store, output_path, band_coun
                                      code=f"{operation}_calculation_failed",
                                      details={"error": str(exc)}) from exc
     finally:
-        if writer is not None:
-            writer.abort()
+        if product is not None:
+            product.suspend()  # the staged product stays for a resume
+
+
+def _sha256(array) -> str | None:
+    import hashlib
+
+    if array is None:
+        return None
+    return hashlib.sha256(np.ascontiguousarray(array).tobytes()).hexdigest()
 
 
 def _vectors(vectors_m, count, name):
@@ -419,8 +454,13 @@ def _run_product(operation: str, plan: DurationPlan, *, sun_vectors_m, earth_vec
                                          tile_y=patch.tile_y, tile_x=patch.tile_x,
                                          valid_height=patch.height, valid_width=patch.width)
 
+    def configuration():
+        return {"plan": plan.identity(), "sun_vectors_sha256": _sha256(state["sun"]),
+                "earth_vectors_sha256": _sha256(state["earth"])}
+
     return _float_product(operation, band_count=plan.bands, descriptions=plan.descriptions,
-                          compute=compute, validate=validate, **kwargs)
+                          compute=compute, validate=validate, configuration=configuration,
+                          **kwargs)
 
 
 def _duration_product(operation, *, sun_signal, sun_threshold, earth_threshold_deg, dem,
@@ -431,7 +471,7 @@ def _duration_product(operation, *, sun_signal, sun_threshold, earth_threshold_d
                       backend: str = "auto", progress_callback: Callable | None = None,
                       progress_event_callback: Callable | None = None,
                       cancellation_requested: Callable | None = None, verbose: bool = False,
-                      device_id: int = 0) -> Path:
+                      device_id: int = 0, start_fresh: bool = False) -> Path:
     if earth_vectors_m is not None and earth_threshold_deg is None:
         raise ValueError("Earth elevation threshold must be finite")
     if earth_vectors_m is None and earth_threshold_deg is not None:
@@ -447,7 +487,7 @@ def _duration_product(operation, *, sun_signal, sun_threshold, earth_threshold_d
                         overwrite=overwrite, backend=backend, progress_callback=progress_callback,
                         progress_event_callback=progress_event_callback,
                         cancellation_requested=cancellation_requested, verbose=verbose,
-                        device_id=device_id)
+                        device_id=device_id, start_fresh=start_fresh)
 
 
 def run_sunlight_duration_product(*, sunlight_fraction_threshold: float, **kwargs) -> Path:
@@ -494,7 +534,7 @@ def run_safe_haven_product(*, dem, horizon_store, output_path, times_utc, sun_ve
                            progress_callback: Callable | None = None,
                            progress_event_callback: Callable | None = None,
                            cancellation_requested: Callable | None = None, verbose: bool = False,
-                           device_id: int = 0) -> Path:
+                           device_id: int = 0, start_fresh: bool = False) -> Path:
     """Per-month safe-haven durations (safe_haven_pipeline.py:49-305): the longest
     low-Sun run overlapping an Earth outage in the month, NaN where ill-posed."""
     plan = safe_haven_plan(times_utc, sunlight_threshold=sunlight_threshold,
@@ -506,7 +546,7 @@ def run_safe_haven_product(*, dem, horizon_store, output_path, times_utc, sun_ve
                         overwrite=overwrite, backend=backend, progress_callback=progress_callback,
                         progress_event_callback=progress_event_callback,
                         cancellation_requested=cancellation_requested, verbose=verbose,
-                        device_id=device_id)
+                        device_id=device_id, start_fresh=start_fresh)
 
 
 def _body_elevation_product(body: str, vectors_m, *, dem, horizon_store, output_path, times_utc,
@@ -515,7 +555,8 @@ def _body_elevation_product(body: str, vectors_m, *, dem, horizon_store, output_
                             progress_callback: Callable | None = None,
                             progress_event_callback: Callable | None = None,
                             cancellation_requested: Callable | None = None,
-                            verbose: bool = False, device_id: int = 0) -> Path:
+                            verbose: bool = False, device_id: int = 0,
+                            start_fresh: bool = False) -> Path:
     """One float32 band per time: the terrain-relative elevation of the body's
     centre, el - lerp(horizon) (elevation_pipeline.py:44-222)."""
     times = tuple(times_utc)
@@ -541,7 +582,8 @@ def _body_elevation_product(body: str, vectors_m, *, dem, horizon_store, output_
                           progress_callback=progress_callback,
                           progress_event_callback=progress_event_callback,
                           cancellation_requested=cancellation_requested, verbose=verbose,
-                          device_id=device_id)
+                          device_id=device_id, start_fresh=start_fresh,
+                          configuration=lambda: {"vectors_sha256": _sha256(state["vectors"])})
 
 
 def run_sun_elevation_product(*, sun_vectors_m, **kwargs) -> Path:

@@ -321,7 +321,10 @@ class GeoTiffWriter:
     def __init__(self, path, width: int, height: int, bands: int, dtype, affine_transform,
                  projection: ProjectionParameters, *, nodata: float | None = None,
                  overwrite: bool = False, compress: bool = True,
-                 descriptions: list[str] | None = None):
+                 descriptions: list[str] | None = None, staging=None, resume=None):
+        """``staging``: a fixed staging file instead of a temporary one (the
+        resumable product store); ``resume``: (offsets, counts, end) of the
+        blocks already in that file, which is truncated to ``end`` and appended to."""
         self.path = Path(path).expanduser().resolve()
         self.dtype = np.dtype(dtype)
         if self.dtype == np.bool_ or self.dtype.kind not in "uif":
@@ -345,11 +348,22 @@ class GeoTiffWriter:
         self.counts = np.zeros(blocks, dtype=np.uint64)
         self.descriptions = descriptions
         self.path.parent.mkdir(parents=True, exist_ok=True)
-        fd, tmp = tempfile.mkstemp(prefix=f".{self.path.name}.", suffix=".tmp.tif",
-                                   dir=self.path.parent)
-        self.tmp = Path(tmp)
-        self.handle = os.fdopen(fd, "wb")
-        self.handle.write(b"\0" * (16 if self.big else 8))  # header, patched on close
+        if staging is None:
+            fd, tmp = tempfile.mkstemp(prefix=f".{self.path.name}.", suffix=".tmp.tif",
+                                       dir=self.path.parent)
+            self.tmp = Path(tmp)
+            self.handle = os.fdopen(fd, "wb")
+        else:
+            self.tmp = Path(staging)
+            self.handle = open(self.tmp, "r+b" if resume is not None else "w+b")
+        if resume is None:
+            self.handle.write(b"\0" * (16 if self.big else 8))  # header, patched on close
+        else:
+            offsets, counts, end = resume
+            self.offsets[:] = np.asarray(offsets, dtype=np.uint64)
+            self.counts[:] = np.asarray(counts, dtype=np.uint64)
+            self.handle.truncate(int(end))
+            self.handle.seek(int(end))
 
     def write_block(self, band: int, by: int, bx: int, block: np.ndarray) -> None:
         """One 128x128 block (smaller edge blocks are zero-padded) of a 0-based band."""
@@ -363,6 +377,15 @@ class GeoTiffWriter:
         self.handle.write(payload)
         if len(payload) % 2:
             self.handle.write(b"\0")
+
+    def block_index(self, band: int, by: int, bx: int) -> int:
+        return (band * self.down + by) * self.across + bx
+
+    def sync(self) -> int:
+        """Flush the staged data to disk; returns the end offset."""
+        self.handle.flush()
+        os.fsync(self.handle.fileno())
+        return self.handle.tell()
 
     def write_region(self, band: int, y0: int, x0: int, values: np.ndarray) -> None:
         """A block-aligned region (y0, x0 multiples of 128) of a 0-based band."""

@@ -122,7 +122,8 @@ class _Progress:
 
 def _run(operation, dem_path, horizons_path, output_path, *, sun_vectors_m, times, backend,
          observer_height_m, invalid_value, compress, overwrite, verbose, progress_callback,
-         progress_event_callback, cancellation_requested, device_id, bands_of, per_patch):
+         progress_event_callback, cancellation_requested, device_id, bands_of, per_patch,
+         start_fresh=False):
     _validate_common(backend, {"progress_callback": progress_callback,
                                "progress_event_callback": progress_event_callback,
                                "cancellation_requested": cancellation_requested}, invalid_value)
@@ -138,39 +139,55 @@ def _run(operation, dem_path, horizons_path, output_path, *, sun_vectors_m, time
                          progress_event_callback)
     store = HorizonTileStore(horizons)
     patches = enumerate_patches(dem.width, dem.height)
-    from .geotiff import GeoTiffWriter
+    from .product_store import ResumableTiledProduct
 
     bands, prepared = bands_of(dem, vectors)
-    writer = None
+    product = None
+
+    def check_cancel():
+        if cancellation_requested is not None and cancellation_requested():
+            raise OperationCancelledError(f"{operation.capitalize()} generation was cancelled.",
+                                          code=f"{operation}_cancelled",
+                                          details={"path": str(output)})
+
+    def blocks(tile, patch):
+        for _, block in per_patch(calculator, dem, tile, prepared, patch):
+            check_cancel()
+            yield block
+
     try:
-        writer = GeoTiffWriter(output, dem.width, dem.height, bands, np.uint8, dem.geo_transform,
-                               dem.projection, overwrite=overwrite, compress=compress,
-                               descriptions=([str(t) for t in times]
-                                             if times is not None and len(times) == bands else None))
-        progress("start", 0, len(patches))
+        import hashlib
+
+        product = ResumableTiledProduct(
+            output, width=dem.width, height=dem.height, bands=bands, dtype=np.uint8,
+            affine_transform=dem.geo_transform, projection=dem.projection, compress=compress,
+            descriptions=([str(t) for t in times]
+                          if times is not None and len(times) == bands else None),
+            algorithm=operation,
+            configuration={"vectors_sha256": hashlib.sha256(vectors.tobytes()).hexdigest(),
+                           "invalid_value": int(invalid_value),
+                           "observer_height_m": float(observer_height_m)},
+            overwrite=overwrite, start_fresh=start_fresh)
+        progress("start", len(product.completed), len(patches))
         for k, patch in enumerate(patches):
-            if cancellation_requested is not None and cancellation_requested():
-                raise OperationCancelledError(f"{operation.capitalize()} generation was cancelled.",
-                                              code=f"{operation}_cancelled",
-                                              details={"path": str(output)})
+            if product.is_complete(patch.tile_y, patch.tile_x):
+                continue
+            check_cancel()
             tile = store.read(patch.tile_y, patch.tile_x, observer_height_m)
-            by, bx = patch.patch_y, patch.patch_x
             if tile is None:
-                block = np.full((patch.height, patch.width), int(invalid_value), np.uint8)
-                for band in range(bands):
-                    writer.write_block(band, by, bx, block)
+                product.write_invalid_patch(patch.tile_y, patch.tile_x, patch.height,
+                                            patch.width, int(invalid_value))
                 progress("invalid", k + 1, len(patches), patch.tile_y, patch.tile_x)
                 continue
             # a ray that met no terrain (-inf in a .bin tile) sees the whole sky
             tile = np.where(np.isfinite(tile), tile, np.float32(-90.0))
-            for band, block in per_patch(calculator, dem, tile, prepared, patch):
-                writer.write_block(band, by, bx, block)
+            product.write_patch(patch.tile_y, patch.tile_x, blocks(tile, patch))
             progress("valid", k + 1, len(patches), patch.tile_y, patch.tile_x)
-        path = writer.close()
-        writer = None
+        path = product.finalize()
+        product = None
         progress("complete", len(patches), len(patches))
         return path
-    except (OperationCancelledError, InputError):
+    except (OperationCancelledError, InputError, ProductGenerationError):
         raise
     except HipExecutionError as exc:
         raise HipError(f"The HIP {operation} backend failed during execution.",
@@ -186,8 +203,8 @@ def _run(operation, dem_path, horizons_path, output_path, *, sun_vectors_m, time
                                      code=f"{operation}_calculation_failed",
                                      details={"error": str(exc)}) from exc
     finally:
-        if writer is not None:
-            writer.abort()
+        if product is not None:
+            product.suspend()  # the staged product stays for a resume
 
 
 def generate_lightmap(dem_path, horizons_path, output_path, *, sun_vectors_m, times=None,
@@ -195,7 +212,8 @@ def generate_lightmap(dem_path, horizons_path, output_path, *, sun_vectors_m, ti
                       invalid_value: int = 0, compress: bool = True, overwrite: bool = False,
                       verbose: bool = False, progress_callback: Callable | None = None,
                       progress_event_callback: Callable | None = None,
-                      cancellation_requested: Callable | None = None, device_id: int = 0) -> Path:
+                      cancellation_requested: Callable | None = None, device_id: int = 0,
+                      start_fresh: bool = False) -> Path:
     """Tiled uint8 visible-solar-fraction GeoTIFF, one band per time (products.py:315-518)."""
 
     def bands_of(dem, vectors):
@@ -213,7 +231,7 @@ def generate_lightmap(dem_path, horizons_path, output_path, *, sun_vectors_m, ti
                 verbose=verbose, progress_callback=progress_callback,
                 progress_event_callback=progress_event_callback,
                 cancellation_requested=cancellation_requested, device_id=device_id,
-                bands_of=bands_of, per_patch=per_patch)
+                bands_of=bands_of, per_patch=per_patch, start_fresh=start_fresh)
 
 
 def generate_psr(dem_path, horizons_path, output_path, *, sun_vectors_m, times=None,
@@ -221,7 +239,8 @@ def generate_psr(dem_path, horizons_path, output_path, *, sun_vectors_m, times=N
                  compress: bool = True, overwrite: bool = False, verbose: bool = False,
                  progress_callback: Callable | None = None,
                  progress_event_callback: Callable | None = None,
-                 cancellation_requested: Callable | None = None, device_id: int = 0) -> Path:
+                 cancellation_requested: Callable | None = None, device_id: int = 0,
+                 start_fresh: bool = False) -> Path:
     """Single-band uint8 PSR classification GeoTIFF, 255 = PSR (products.py:520-698).
 
     The Sun vectors are first reduced to the per-azimuth-bin maxima seen from
@@ -244,4 +263,4 @@ def generate_psr(dem_path, horizons_path, output_path, *, sun_vectors_m, times=N
                 verbose=verbose, progress_callback=progress_callback,
                 progress_event_callback=progress_event_callback,
                 cancellation_requested=cancellation_requested, device_id=device_id,
-                bands_of=bands_of, per_patch=per_patch)
+                bands_of=bands_of, per_patch=per_patch, start_fresh=start_fresh)

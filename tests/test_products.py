@@ -96,4 +96,52 @@ def test_product_errors(site, tmp_path):
     with pytest.raises(OperationCancelledError):
         generate_lightmap(dem_path, horizons, tmp_path / "c.tif", sun_vectors_m=sun, backend="cpu",
                           cancellation_requested=lambda: True)
-    assert not (tmp_path / "c.tif").exists() and not list(tmp_path.glob(".c.tif.*"))
+    # cancelled: no output, the staged product stays for a resume (product_store.py:255-303)
+    assert not (tmp_path / "c.tif").exists()
+    assert len(list(tmp_path.glob(".c.tif.lunarscout-partial.*"))) == 3
+
+
+def test_products_resume_after_cancellation(site, tmp_path):
+    from lunarscout_amd.product_store import IncompatibleProductJobError
+
+    dem, dem_path, horizons, written, sun = site
+    calls = {"n": 0}
+
+    def cancel_on_third_patch():
+        calls["n"] += 1
+        return calls["n"] > 20  # inside the second patch (one check per patch and per band)
+
+    with pytest.raises(OperationCancelledError):
+        generate_lightmap(dem_path, horizons, tmp_path / "r.tif", sun_vectors_m=sun[:12],
+                          backend="cpu", cancellation_requested=cancel_on_third_patch)
+    journal = tmp_path / ".r.tif.lunarscout-partial.journal.jsonl"
+    done = journal.read_text().count("\n")
+    assert 0 < done < 4
+    # a torn last journal line is dropped on resume
+    with open(journal, "a") as handle:
+        handle.write('{"patch": "0,0", "sta')
+    # another job's staged product is refused
+    with pytest.raises(IncompatibleProductJobError) as info:
+        generate_lightmap(dem_path, horizons, tmp_path / "r.tif", sun_vectors_m=sun[:11],
+                          backend="cpu")
+    assert info.value.code == "product_job_incompatible"
+    events = []
+    resumed = generate_lightmap(dem_path, horizons, tmp_path / "r.tif", sun_vectors_m=sun[:12],
+                                backend="cpu", progress_event_callback=events.append)
+    assert events[0].completed == done
+    fresh = generate_lightmap(dem_path, horizons, tmp_path / "f.tif", sun_vectors_m=sun[:12],
+                              backend="cpu")
+    for band in (1, 7, 12):
+        a, _ = geotiff.read_geotiff(resumed, band=band)
+        b, _ = geotiff.read_geotiff(fresh, band=band)
+        np.testing.assert_array_equal(a, b)
+    assert not list(tmp_path.glob(".r.tif.lunarscout-partial.*"))
+    # start_fresh discards a staged product of another job
+    with pytest.raises(OperationCancelledError):
+        generate_lightmap(dem_path, horizons, tmp_path / "s.tif", sun_vectors_m=sun[:5],
+                          backend="cpu", cancellation_requested=lambda: True)
+    out = generate_lightmap(dem_path, horizons, tmp_path / "s.tif", sun_vectors_m=sun[:12],
+                            backend="cpu", start_fresh=True)
+    a, _ = geotiff.read_geotiff(out, band=12)
+    b, _ = geotiff.read_geotiff(fresh, band=12)
+    np.testing.assert_array_equal(a, b)
