@@ -148,10 +148,15 @@ def main() -> int:
     lighting_ms = stats["lighting_ms"] / max(stats["lighting_launches"], 1)
     light_px_times = per_step * PIXELS * (len(sun) + len(earth) + len(psr_vectors))
 
+    # the resident horizons are those of the last step's tiles
+    last = args.warmup + args.steps - 1
+    resident = [tiles[(last * per_step + i) % len(tiles)] for i in range(per_step)]
+
     # traversal counters on the timed tiles -> algorithmic bytes per tile
     S = T = bytes_alg = achieved = None
     if args.counter_tiles > 0:
         counters = DeviceBuffer.empty(session, (4,), np.int64)
+        scratch = DeviceBuffer.empty(session, (PIXELS, AZIMUTHS), np.float32)
         samples = tests = 0
         timed = tiles[args.warmup * per_step:(args.warmup + args.steps) * per_step] or tiles
         pick = [timed[i] for i in np.linspace(0, len(timed) - 1, args.counter_tiles).astype(int)]
@@ -159,10 +164,12 @@ def main() -> int:
             counters.zero()
             segments = session.segments_device(dems, tile_column=c, tile_row=r, out=segments)
             session.march_device(dems, segments, tile_column=c, tile_row=r, flags=HORIZON_COUNTERS,
-                                 output=_slot_view(horizons, 0), counters=counters)
+                                 output=scratch, counters=counters)
             v = counters.to_host()
             samples += int(v[0])
             tests += int(v[1])
+        scratch.free()
+        counters.free()
         rays = PIXELS * AZIMUTHS
         S, T = samples / len(pick), tests / len(pick)
         passes = len(dems)
@@ -172,8 +179,8 @@ def main() -> int:
     codec = _codec_leg(horizons)
     long_lighting = reducers = None
     if args.year_lighting:
-        long_lighting = _year_lighting(session, lighting, primary, horizons, tiles[:per_step])
-        reducers = _durations_leg(session, lighting, primary, horizons, tiles[:per_step])
+        long_lighting = _year_lighting(session, lighting, primary, horizons, resident)
+        reducers = _durations_leg(session, lighting, primary, horizons, resident)
     peaks = None
     if not args.no_probes:
         from lunarscout_amd.profiling import probe_peaks
