@@ -360,7 +360,10 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     const S fallback = (lsh_ldexp(res, 2 * level) * S(0.5)) * S(0.001);
     const S exit_distance = lsh_cell_exit(static_cast<S>(cx * scale - g.bx) - px,
                                           static_cast<S>(cy * scale - g.by) - py, fscale, dx, dy);
-    const S skip = exit_distance > S(0) ? exit_distance + S(0.0001) : fallback;
+    // the distance to the cell exit, or the fallback when the exit is not ahead
+    // (cuda_backend.py:756-763): a cull or nodata skip moves 1e-4 km past it
+    const S to_exit = exit_distance > S(0) ? exit_distance : fallback;
+    const S skip = to_exit + S(0.0001);
     S advance = skip;
     if (sampled) {
       // bilinear (cuda_backend.py:107-132) of h - oz on the level-0 taps of the
@@ -407,7 +410,7 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
       if (near) step = step * S(0.25);
       const S floor_step = (pass == 0 && true_m >= S(100)) ? far_step : min_step;
       step = lsh_max(step, floor_step);
-      if (kCap) step = lsh_min(step, exit_distance > S(0) ? exit_distance + S(0.000001) : fallback);
+      if (kCap) step = lsh_min(step, to_exit + S(0.000001));  // cuda_backend.py:636-640
       advance = step;
       trace.row(float(s.value()), float(true_m), 0, cx, cy, float(px + g.bx), float(py + g.by),
                 maxh, float(hd), float(slope), float(advance), 4);

@@ -52,6 +52,23 @@ def horizon_tile(pyramids, segment_values, *, tile_column, tile_row, tile_width=
     return out, win, counters
 
 
+def march_segment(segment, pyramid, *, observer_z, pass_index=0, initial=-np.inf, mode=6,
+                  cap=False, capacity=100000):
+    """The kernel core's pass over one raw segment: (slope, trace rows [n, 12])."""
+    keep = oracle._Keep()
+    seg = keep.add(np.ascontiguousarray(segment, dtype=np.float32))
+    p = pyramid.struct(keep)
+    rows = np.zeros((capacity, 12), dtype=np.float32)
+    res = ctypes.c_double()
+    v = ctypes.c_void_p
+    n = lib().kcpu_march_segment(
+        v(seg.ctypes.data), ctypes.byref(p), ctypes.c_double(observer_z),
+        ctypes.c_int32(pass_index), ctypes.c_double(initial),
+        ctypes.c_int32((1 if cap else 0) | mode), v(rows.ctypes.data), ctypes.c_int32(capacity),
+        ctypes.byref(res))
+    return res.value, rows[:min(n, capacity)].copy()
+
+
 def trace_ray(pyramids, segment_values, *, tile_column, tile_row, pixel, azimuth,
               tile_width=128, subpatch_size=8, observer_elevation_m=0.0, capacity=200000,
               mode=0):

@@ -235,4 +235,44 @@ int kcpu_trace_ray(const kc_pyramid* pyrs, int32_t D, const float* segments, int
   return n;
 }
 
+// hierarchy_maximum over one raw 18-field segment (no subpatch blend), the
+// form oracle.march_ray takes: the kernel core's pass with the segment used as
+// all four blend corners at zero weight (exact), so constructed rays can be
+// compared with the oracle trace row for trace row. flags: 1 = cap,
+// 2 = float64 schedule, 4 = float64 values. Returns the row count.
+int kcpu_march_segment(const float* segment, const kc_pyramid* pyr, double observer_z,
+                       int32_t pass_index, double initial, int32_t flags, float* rows,
+                       int32_t cap, double* result) {
+  PassTables t = tables(pyr, 1);
+  lsh::PixelSelection sel{0, 0, 0, 0, 0.0f, 0.0f, 0, 0, 0, 0};
+  lsh::MarchView v{t.combined[0].data(), uint32_t(4 * t.combined[0].size()), t.bound_offset[0],
+                   t.off[0], t.lw[0], t.lh[0], pyr->dem.width, pyr->dem.height,
+                   pyr->level_count, t.mapres[0], float(pyr->dem.projection[0]),
+                   t.all_valid[0], double(t.mapres[0]), res_exponent(t.mapres[0])};
+  lsh::MarchCounters cnt{0, 0, 0};
+  int n = 0;
+  RowTrace tr{{}, rows, cap, &n};
+  int32_t best = -1;
+  auto run = [&](auto s_tag, auto v_tag) {
+    using S = decltype(s_tag);
+    using V = decltype(v_tag);
+    lsh::PassSegment<S> g =
+        lsh::lsh_interpolate<S>(segment, segment, segment, segment, sel, 0, 0, 1.0f);
+    V current = __builtin_isinf(initial) ? V(-1e30) : V(initial);
+    if (flags & 1)
+      lsh::lsh_march_pass<S, V, true, true>(g, v, V(observer_z), pass_index, current, best, cnt,
+                                            tr);
+    else
+      lsh::lsh_march_pass<S, V, false, true>(g, v, V(observer_z), pass_index, current, best,
+                                             cnt, tr);
+    *result = current > V(-1e29) ? double(current) : -std::numeric_limits<double>::infinity();
+  };
+  switch (flags & 6) {
+    case 0: run(0.0f, 0.0f); break;
+    case 2: run(0.0, 0.0f); break;
+    default: run(0.0, 0.0); break;
+  }
+  return n;
+}
+
 }  // extern "C"
