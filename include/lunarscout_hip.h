@@ -38,6 +38,9 @@ extern "C" {
 #define LSH_HORIZON_COUNTERS 4  /* accumulate traversal counters (hierarchy.py:40-52) */
 #define LSH_HORIZON_F32_VALUES 8 /* heights, bounds and slopes in float32 (faster; float32 ties
                                     let rare rays drift from the float64 method) */
+#define LSH_HORIZON_AZIMUTH_MAJOR 16 /* output [N][tile_height*tile_width] instead of the
+                                        pixel-major tile file order (what the lighting kernel
+                                        reads coalesced; lsh_transpose_tiles converts) */
 
 /* lsh_lighting_tiles flags */
 #define LSH_LIGHT_SUN_BYTES 1     /* per-time uint8 sun fraction bytes */
@@ -131,7 +134,8 @@ int lsh_segments_tile(lsh_session* session, const int32_t* dem_ids, int32_t dem_
  * carried in registers (CudaSession.subpatch_hierarchical_all_passes,
  * cuda_backend.py:1445-1552, and subpatch_hierarchy_kernel,
  * cuda_backend.py:1007-1134). segments (device): float32[N][C][D][18];
- * output (device): float32[tile_height*tile_width][N], pixel-major; pixels
+ * output (device): float32[tile_height*tile_width][N], pixel-major (or
+ * [N][tile_height*tile_width] with LSH_HORIZON_AZIMUTH_MAJOR); pixels
  * outside the primary DEM are -inf. counters (device, nullable, with
  * LSH_HORIZON_COUNTERS): int64[4] += (level-0 samples, bound tests, steps,
  * 64 x the longest lane's steps per wavefront). */
@@ -164,8 +168,9 @@ int lsh_horizon_forward(lsh_session* session, const int32_t* dem_ids, int32_t de
  * fraction (lightmap.py:32-73) with each slice clamp rounded over
  * +-smoothing_deg (C1). loss (device, nullable) float64 [tile][16384];
  * grad_horizon (device, nullable) float32 [tile][16384][N]. Inputs as
- * lsh_lighting_tiles (pixel-major horizons). smoothing_deg must not exceed the
- * narrowest slice half-height (0.094 deg for the reference disk). */
+ * lsh_lighting_tiles (pixel-major horizons), 2 <= N <= 1440; the slice walk
+ * scales the reference's 1440-bin constants to N bins. smoothing_deg must not
+ * exceed the narrowest slice half-height (0.094 deg for the reference disk). */
 int lsh_sun_loss(lsh_session* session, int32_t dem_id, const lsh_tile* tiles,
                  int32_t tile_count, const float* horizons, int32_t azimuth_count,
                  const double* sun, int32_t sun_count, double sun_half_angle_deg,
@@ -225,7 +230,8 @@ int lsh_batch_create(lsh_session* session, const lsh_batch_site* sites, int32_t 
  * launch: `workgroups` resident workgroups (a multiple of 8; <= 0 picks one
  * per the kernel's occupancy) drain per-XCD work queues of (azimuth group,
  * tile, pixel block) items, an XCD first taking the items of its own angular
- * sector and then stealing. flags: 0 or LSH_HORIZON_DEGREES. use_graph = 1
+ * sector and then stealing. flags: LSH_HORIZON_DEGREES and/or
+ * LSH_HORIZON_AZIMUTH_MAJOR (each tile slot [N][16384]). use_graph = 1
  * replays the captured sequence (captured again when tile_count, config,
  * workgroups, flags or output change). Stream ordered; the call returns after
  * enqueueing (lsh_synchronize waits). */
@@ -243,7 +249,9 @@ int lsh_batch_destroy(lsh_batch* batch);
  * azimuth_major = 1, [tile][N][16384]. sun / earth / psr_vectors (device):
  * float64 Moon-ME positions [count][3] (nullable when the count is 0).
  * sun_half_angle_deg: the disk half angle of the 16-slice sun (reference 0.27);
- * psr_diameter_deg: the PSR upper-limb diameter (reference 0.545).
+ * psr_diameter_deg: the PSR upper-limb diameter (reference 0.545). The slice
+ * walk is the reference's at N = 1440 and scales its bin constants to N bins
+ * otherwise (N >= 2).
  * Outputs (device, each nullable, selected by flags LSH_LIGHT_*):
  *   sun_bytes uint8 [tile][sun_count][16384] = uint8(255 * fraction)
  *   sun_fraction float [tile][sun_count][16384]; sun_mean float [tile][16384]

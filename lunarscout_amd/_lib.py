@@ -27,6 +27,7 @@ HORIZON_DEGREES = 1
 HORIZON_CAP = 2
 HORIZON_COUNTERS = 4
 HORIZON_F32_VALUES = 8
+HORIZON_AZIMUTH_MAJOR = 16
 
 LIGHT_SUN_BYTES = 1
 LIGHT_SUN_FRACTION = 2
@@ -226,6 +227,20 @@ class DeviceBuffer:
     def zero(self) -> None:
         check(load_library().lsh_memset_device(self.session.handle, self.ptr, 0, self.nbytes),
               "lsh_memset_device")
+
+    def slot(self, index: int) -> "DeviceBuffer":
+        """A view of sub-array ``index`` along the first axis (the parent owns the memory)."""
+        if not 0 <= index < self.shape[0]:
+            raise ContractValidationError("slot index out of range")
+        view = DeviceBuffer.__new__(DeviceBuffer)
+        view.session = self.session
+        view.dtype = self.dtype
+        view.shape = self.shape[1:]
+        view.nbytes = int(np.prod(view.shape, dtype=np.int64)) * view.dtype.itemsize
+        view.ptr = self.ptr + index * view.nbytes
+        view._parent = self
+        view.free = lambda: None
+        return view
 
     def free(self) -> None:
         if self.ptr and self.session.handle:

@@ -660,6 +660,8 @@ int lsh_sun_loss(lsh_session* session, int32_t dem_id, const lsh_tile* tiles,
   LSH_REQUIRE(d, "unknown DEM id");
   LSH_REQUIRE(loss || grad_horizon, "nothing to compute");
   LSH_REQUIRE(sun_count >= 1, "sun vectors are required");
+  // the gradient keeps the <= 5 bins one time's slice walk touches at N <= 1440
+  LSH_REQUIRE(azimuth_count <= 1440, "the sun loss takes at most 1440 azimuths");
   lsh::LightArgs a;
   if (int r = light_args(tiles, tile_count, d->width, d->height, horizons, azimuth_count, 0, sun,
                          sun_count, nullptr, 0, nullptr, 0, sun_half_angle_deg, 0.545, 0,

@@ -90,7 +90,8 @@ __global__ void __launch_bounds__(256) sun_loss_kernel(LossArgs a) {
     azimuth_elevation(f, L.sun + 3 * t, az, el);
     // the slice walk of sun_fraction (lightmap.py:32-73), float32 positions
     const float el_f = static_cast<float>(el);
-    float left_pos = (static_cast<float>(az) - L.sun_half_angle - 0.125f) * 4.0f;
+    const DiskWalk w = disk_walk(N);
+    float left_pos = (static_cast<float>(az) - L.sun_half_angle - w.half_bucket) * w.per_degree;
     int left = static_cast<int>(left_pos);
     float frac = left_pos - static_cast<float>(left);
     if (left < 0) left += N;
@@ -100,7 +101,7 @@ __global__ void __launch_bounds__(256) sun_loss_kernel(LossArgs a) {
     int lo_bin = left;
     double photons = 0.0;
     // gradient of this time, kept per bin the slices touch: the walk spans
-    // frac + 15 * 0.135 < 3.03 bins, so at most 5 bins
+    // frac + 15 * step < 3.03 bins (step <= 0.135 for N <= 1440), so at most 5 bins
     double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     int bins[5] = {left, right, 0, 0, 0};
     int used = 2;
@@ -118,7 +119,7 @@ __global__ void __launch_bounds__(256) sun_loss_kernel(LossArgs a) {
         g[slot_lo] -= slope * (1.0 - static_cast<double>(frac));
         g[slot_hi] -= slope * static_cast<double>(frac);
       }
-      frac = frac + 0.135f;
+      frac = frac + w.step;
       if (frac >= 1.0f) {
         lo_bin = right;
         right += 1;

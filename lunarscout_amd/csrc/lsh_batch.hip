@@ -184,7 +184,8 @@ int lsh_batch_run(lsh_batch* b, const lsh_batch_tile* tiles, int32_t tile_count,
   LSH_REQUIRE(b && tiles && output, "invalid arguments");
   LSH_REQUIRE(tile_count >= 1 && tile_count <= b->max_tiles, "tile_count exceeds the batch size");
   LSH_REQUIRE(config >= 0 && config < lsh::batch_shape_count(), "unknown kernel shape");
-  LSH_REQUIRE((flags & ~LSH_HORIZON_DEGREES) == 0, "batch flags: 0 or LSH_HORIZON_DEGREES");
+  LSH_REQUIRE((flags & ~(LSH_HORIZON_DEGREES | LSH_HORIZON_AZIMUTH_MAJOR)) == 0,
+              "batch flags: LSH_HORIZON_DEGREES and/or LSH_HORIZON_AZIMUTH_MAJOR");
   LSH_REQUIRE(workgroups <= 0 || workgroups % 8 == 0, "workgroups must be a multiple of 8");
   for (int i = 0; i < tile_count; ++i) {
     const lsh_batch_tile& t = tiles[i];

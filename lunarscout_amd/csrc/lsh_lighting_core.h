@@ -107,11 +107,25 @@ LSH_LHD double horizon_at(const Row& hz, int N, double az) {
   return lo + f * (static_cast<double>(hz[upper]) - lo);
 }
 
+// The disk walk's scale for an N-bin horizon row: bins per degree, half a bin
+// in degrees and the slice spacing in bins. The reference walks 1440 bins with
+// the constants x4.0, 0.125 and 0.135 (lightmap.py:15-29, 39-43), which these
+// are exactly at N = 1440; another N scales them to its bin width, so the walk
+// never leaves the row (a fixed x4.0 would index up to 4 x 360 bins).
+struct DiskWalk {
+  float per_degree, half_bucket, step;
+};
+LSH_LHD DiskWalk disk_walk(int N) {
+  const float b = static_cast<float>(N) / 360.0f;
+  return DiskWalk{b, 0.5f / b, 0.135f * (b * 0.25f)};
+}
+
 // BuilderSunFraction (lightmap.py:32-73) in float32.
 template <class Row>
 LSH_LHD float sun_fraction(const Row& hz, int N, float az_deg, float el_deg,
                                      const LightArgs& a) {
-  float left_pos = (az_deg - a.sun_half_angle - 0.125f) * 4.0f;
+  const DiskWalk w = disk_walk(N);
+  float left_pos = (az_deg - a.sun_half_angle - w.half_bucket) * w.per_degree;
   int left = static_cast<int>(left_pos);  // truncation toward zero
   float frac = left_pos - static_cast<float>(left);
   if (left < 0) left += N;
@@ -132,7 +146,7 @@ LSH_LHD float sun_fraction(const Row& hz, int N, float az_deg, float el_deg,
       if (ad > ch) ad = ch;
       photons = photons + ad;
     }
-    frac = frac + 0.135f;
+    frac = frac + w.step;
     if (frac >= 1.0f) {
       right += 1;
       if (right >= N) right = 0;

@@ -50,11 +50,15 @@ template <class V, bool kCap, bool kCount>
 __device__ __attribute__((always_inline)) void march_pixel(
     const MarchDem* dems, const int32_t (*s_off)[kMaxLevels], const int32_t (*s_w)[kMaxLevels],
     const int32_t (*s_h)[kMaxLevels], int D, const float* segments, int N, int C,
-    int tile_column, int tile_row, int tile_width, int subpatch, float observer_elevation,
-    int flags, int column, int row, int az, float* output, int32_t* winners,
-    unsigned long long* counters) {
+    int tile_column, int tile_row, int tile_width, int tile_height, int subpatch,
+    float observer_elevation, int flags, int column, int row, int az, float* output,
+    int32_t* winners, unsigned long long* counters) {
   const int pixel = row * tile_width + column;
-  const int64_t out_index = static_cast<int64_t>(pixel) * N + az;
+  // pixel-major (the tile file order) or azimuth-major: a wavefront (one
+  // azimuth, 8 rows of 8 pixels) then stores 8 contiguous 32-byte runs
+  const int64_t out_index = (flags & LSH_HORIZON_AZIMUTH_MAJOR)
+                                ? static_cast<int64_t>(az) * (tile_width * tile_height) + pixel
+                                : static_cast<int64_t>(pixel) * N + az;
   const MarchDem& primary = dems[0];
   if (tile_column + column >= primary.width || tile_row + row >= primary.height) {
     output[out_index] = -INFINITY;  // padded pixels keep the sentinel
@@ -143,8 +147,9 @@ march_kernel(MarchArgs args) {
   if (az >= args.N || column >= args.tile_width || row >= args.tile_height) return;
   march_pixel<V, kCap, kCount>(args.dem, s_off, s_w, s_h, args.D, args.segments, args.N,
                                args.C, args.tile_column, args.tile_row, args.tile_width,
-                               args.subpatch, args.observer_elevation, args.flags, column, row,
-                               az, args.output, args.winners, args.counters);
+                               args.tile_height, args.subpatch, args.observer_elevation,
+                               args.flags, column, row, az, args.output, args.winners,
+                               args.counters);
 }
 
 // The persistent batched march. Work items are (azimuth group, tile, pixel
@@ -208,8 +213,8 @@ march_batch_kernel(BatchMarchArgs a) {
     if (az < a.N) {
       march_pixel<double, false, false>(ms.dem, s_off, s_w, s_h, ms.D,
                                  a.segments + tile * a.slot_floats, a.N, a.C, t.tile_column,
-                                 t.tile_row, 128, a.subpatch, ms.observer_elevation, a.flags,
-                                 column, row, az,
+                                 t.tile_row, 128, 128, a.subpatch, ms.observer_elevation,
+                                 a.flags, column, row, az,
                                  a.output + static_cast<int64_t>(tile) * 128 * 128 * a.N,
                                  nullptr, nullptr);
     }

@@ -160,12 +160,20 @@ class HipSession(_lib.Session):
                      tile_row: int, tile_width: int = PATCH_SIZE, tile_height: int = PATCH_SIZE,
                      subpatch_size: int = SUBPATCH_SIZE, observer_elevation_m: float = 0.0,
                      flags: int = 0, output: DeviceBuffer | None = None,
-                     counters: DeviceBuffer | None = None) -> DeviceBuffer:
+                     counters: DeviceBuffer | None = None,
+                     azimuth_major: bool = False) -> DeviceBuffer:
+        """March one tile into ``output``: float32 [pixels, N], or [N, pixels] azimuth-major."""
         ids = self._ids(dems)
         azimuth_count = segments.shape[0]
-        shape = (tile_width * tile_height, azimuth_count)
-        if output is None or output.shape != shape:
+        pixels = tile_width * tile_height
+        if azimuth_major:
+            flags |= _lib.HORIZON_AZIMUTH_MAJOR
+        shape = (azimuth_count, pixels) if flags & _lib.HORIZON_AZIMUTH_MAJOR else (pixels, azimuth_count)
+        if output is None:
             output = DeviceBuffer.empty(self, shape, np.float32)
+        elif output.shape != shape or output.dtype != np.float32:
+            raise ContractValidationError(
+                f"march output must be float32 {shape}, got {output.dtype} {output.shape}")
         check(self.lib.lsh_horizon_tile(
             self.handle, ids.ctypes.data, len(ids), segments.ptr, azimuth_count, tile_column,
             tile_row, tile_width, tile_height, subpatch_size, float(observer_elevation_m),
@@ -236,6 +244,9 @@ class HipSession(_lib.Session):
             counter_buf = DeviceBuffer.empty(self, (4,), np.int64)
             counter_buf.zero()
             flags |= _lib.HORIZON_COUNTERS
+        if self._output is not None and self._output.shape != (PATCH_SIZE * PATCH_SIZE, azimuth_count):
+            self._output.free()
+            self._output = None
         self._output = self.march_device(dems, self._segments, tile_column=tile_column,
                                          tile_row=tile_row,
                                          observer_elevation_m=observer_elevation_m,

@@ -9,6 +9,10 @@ Command line:
 * ``python -m lunarscout_amd.profiling copy --mib M`` runs one device copy of
   M MiB (the known-size stream that scales rocprofv3's FETCH_SIZE and
   WRITE_SIZE counters into bytes);
+* ``python -m lunarscout_amd.profiling workload --tiles K --times T`` runs the
+  full-site march on K tiles (segments + one march launch each, degrees) and,
+  with T > 0, one lighting launch over them with T Sun + T Earth vectors
+  (2-hour steps) plus their PSR reduction -- a short program for counter runs;
 * ``python -m lunarscout_amd.profiling summarize ...`` turns rocprofv3 CSV
   output (kernel trace stats and the FETCH_SIZE / WRITE_SIZE passes) into the
   short text and JSON summaries kept with the sources.
@@ -78,6 +82,47 @@ def _copy(mib: int) -> int:
     check(load_library().lsh_probe_copy(session.handle, dst.ptr, src.ptr, nbytes), "lsh_probe_copy")
     session.synchronize()
     print(json.dumps({"copy_bytes": nbytes}), flush=True)
+    return 0
+
+
+def _workload(args) -> int:
+    from . import synthetic
+    from ._lib import HORIZON_DEGREES, DeviceBuffer
+    from .hip_backend import HipSession
+    from .lighting import HipLightingSession, reduce_sun_vectors_for_psr
+
+    dems = synthetic.site_dems(synthetic.FULL_SITE)
+    primary = dems[0]
+    session = HipSession(0)
+    grid = [(x, y) for y in range(0, primary.height, 128) for x in range(0, primary.width, 128)]
+    tiles = [grid[(i * 97) % len(grid)] for i in range(args.tiles)]
+    layout = "azimuth-major" if args.azimuth_major else "pixel-major"
+    shape = (len(tiles), 1440, 128 * 128) if args.azimuth_major else (len(tiles), 128 * 128, 1440)
+    horizons = DeviceBuffer.empty(session, shape, np.float32)
+    segments = None
+    for slot, (c, r) in enumerate(tiles):
+        segments = session.segments_device(dems, tile_column=c, tile_row=r, out=segments)
+        session.march_device(dems, segments, tile_column=c, tile_row=r, flags=HORIZON_DEGREES,
+                             output=horizons.slot(slot), azimuth_major=args.azimuth_major)
+        print(f"march tile {slot + 1}/{len(tiles)} ({c}, {r}) enqueued", flush=True)
+    session.synchronize()
+    if args.times > 0:
+        lighting = HipLightingSession(session)
+        sun = synthetic.sun_vectors(primary, args.times, 2.0, seed=5)
+        earth = synthetic.earth_vectors(args.times, 2.0, seed=6)
+        psr, _ = reduce_sun_vectors_for_psr(primary, sun)
+        bufs = [DeviceBuffer.from_host(session, v) for v in (sun, earth, psr)]
+        out = lighting.lighting_tiles(primary, tiles, horizons, sun=bufs[0], earth=bufs[1],
+                                      psr_vectors=bufs[2], sun_bytes=True, sun_mean=True,
+                                      earth_visible=True, psr=True,
+                                      azimuth_major=args.azimuth_major)
+        session.synchronize()
+        out.free()
+        print(f"lighting: {len(tiles)} tiles x ({len(sun)} + {len(earth)} + {len(psr)}) vectors, "
+              f"{layout}", flush=True)
+    print(json.dumps({"tiles": tiles, "times": args.times, "layout": layout}), flush=True)
+    horizons.free()
+    session.close()
     return 0
 
 
@@ -186,6 +231,11 @@ def main(argv=None) -> int:
     sub = p.add_subparsers(dest="command", required=True)
     c = sub.add_parser("copy", help="one device copy of --mib MiB")
     c.add_argument("--mib", type=int, default=1024)
+    w = sub.add_parser("workload", help="full-site march tiles and one lighting launch")
+    w.add_argument("--tiles", type=int, default=4)
+    w.add_argument("--times", type=int, default=4380)
+    w.add_argument("--azimuth-major", action="store_true",
+                   help="march into and light from [azimuth][pixel] horizons")
     s = sub.add_parser("summarize", help="summarize rocprofv3 CSV output")
     s.add_argument("--trace", required=True, help="directory of the --kernel-trace --stats run")
     s.add_argument("--trace-command", required=True)
@@ -203,6 +253,8 @@ def main(argv=None) -> int:
     args = p.parse_args(argv)
     if args.command == "copy":
         return _copy(args.mib)
+    if args.command == "workload":
+        return _workload(args)
     return summarize(args)
 
 

@@ -502,7 +502,11 @@ def smoothed_sun_loss(dem, horizons_deg, vectors, *, tile_y: int, tile_x: int, p
         row, column = tile_y + int(p) // 128, tile_x + int(p) % 128
         az, el = azimuth_elevation(dem, row, column, vectors)
         el_f = el.astype(np.float32).astype(np.float64)
-        left_pos = ((az.astype(np.float32) - np.float32(half_angle_deg)) - np.float32(0.125)) * np.float32(4.0)
+        # bins per degree, half a bin and the slice step: x4.0, 0.125, 0.135 at 1440 bins
+        per_degree = np.float32(N) / np.float32(360.0)
+        half_bucket = np.float32(0.5) / per_degree
+        slice_step = np.float32(0.135) * (per_degree * np.float32(0.25))
+        left_pos = ((az.astype(np.float32) - np.float32(half_angle_deg)) - half_bucket) * per_degree
         left = np.trunc(left_pos).astype(np.int64)
         frac = (left_pos - left.astype(np.float32)).astype(np.float32)
         left = np.where(left < 0, left + N, np.where(left >= N, left - N, left))
@@ -513,7 +517,7 @@ def smoothed_sun_loss(dem, horizons_deg, vectors, *, tile_y: int, tile_x: int, p
             c = float(half[i])
             horizon = frac.astype(np.float64) * (hz[k, right] - hz[k, lo]) + hz[k, lo]
             photons += _smooth_clamp(el_f + c - horizon, 2.0 * c, smoothing_deg)
-            frac = (frac + np.float32(0.135)).astype(np.float32)
+            frac = (frac + slice_step).astype(np.float32)
             step = frac >= np.float32(1.0)
             lo = np.where(step, right, lo)
             right = np.where(step, np.where(right + 1 >= N, 0, right + 1), right)

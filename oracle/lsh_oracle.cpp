@@ -844,9 +844,17 @@ const Disk kDisk;
 
 // _sun_fraction_reference (lightmap.py:32-73): C# BuilderSunFraction. T = float
 // reproduces the float32 accumulation; T = double evaluates the same slices in float64.
+//
+// The reference walks 1440 bins: x4.0 bins per degree, half a bin = 0.125 deg,
+// slices 0.135 bins apart. Another N scales the three to its bin width (equal
+// to the reference's constants, bit for bit, at N = 1440).
 template <class T>
 T sun_fraction(const float* hz, int N, T az_deg, T el_deg) {
-  T left_pos = (az_deg - T(0.27f) - T(0.125f)) * T(4.0);
+  const float bins_per_degree = float(N) / 360.0f;
+  const T per_degree = T(bins_per_degree);
+  const T half_bucket = T(0.5f / bins_per_degree);
+  const T step = T(0.135f * (bins_per_degree * 0.25f));
+  T left_pos = (az_deg - T(0.27f) - half_bucket) * per_degree;
   int left = static_cast<int>(left_pos);  // truncation toward zero
   T frac = left_pos - T(left);
   if (left < 0) left += N;
@@ -866,7 +874,7 @@ T sun_fraction(const float* hz, int N, T az_deg, T el_deg) {
       if (ad > ch) ad = ch;
       photons = photons + ad;
     }
-    frac = frac + T(0.135f);
+    frac = frac + step;
     if (frac >= T(1)) {
       left = right;
       right += 1;

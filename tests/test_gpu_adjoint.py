@@ -64,7 +64,11 @@ def test_loss_and_horizon_gradient_match_a_pytorch_reference():
             h = torch.tensor(np.maximum(horizons[tile_index, p].astype(np.float64), -90.0),
                              requires_grad=True)
             # the slice walk's bins and float32 fractions (index work, no gradient)
-            left_pos = ((az.astype(np.float32) - np.float32(0.27)) - np.float32(0.125)) * np.float32(4)
+            # N bins: x N/360 bins per degree, half a bin, slices 0.135 * (N / 1440) bins apart
+            per_degree = np.float32(N) / np.float32(360.0)
+            half_bucket = np.float32(0.5) / per_degree
+            slice_step = np.float32(0.135) * (per_degree * np.float32(0.25))
+            left_pos = ((az.astype(np.float32) - np.float32(0.27)) - half_bucket) * per_degree
             left = np.trunc(left_pos).astype(np.int64)
             frac = (left_pos - left.astype(np.float32)).astype(np.float32)
             left = np.where(left < 0, left + N, np.where(left >= N, left - N, left))
@@ -83,7 +87,7 @@ def test_loss_and_horizon_gradient_match_a_pytorch_reference():
                         torch.where(x <= W - s, x,
                         torch.where(x < W + s, W - (W + s - x) ** 2 / (4 * s), torch.full_like(x, W)))))
                 total = total + value.sum()
-                frac = (frac + np.float32(0.135)).astype(np.float32)
+                frac = (frac + slice_step).astype(np.float32)
                 step = frac >= np.float32(1.0)
                 lo = np.where(step, right, lo)
                 right = np.where(step, np.where(right + 1 >= N, 0, right + 1), right)
