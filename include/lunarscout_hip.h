@@ -25,12 +25,14 @@ extern "C" {
 /* Status codes. LSH_EINVAL maps to ValueError (cuda_backend.py:1459-1465);
  * LSH_ENODEV to the backend-unavailable error (CudaBackendError,
  * cuda_backend.py:1164-1176); LSH_EDEVICE to an execution failure
- * (horizon.py:426-431). */
+ * (horizon.py:426-431); LSH_EIO to the OSError a failed file write raises
+ * (file_format.py:398-415). */
 #define LSH_OK 0
 #define LSH_EINVAL -1
 #define LSH_ENODEV -2
 #define LSH_EDEVICE -3
 #define LSH_ENOMEM -4
+#define LSH_EIO -5
 
 /* lsh_horizon_tile / lsh_horizon_batch flags */
 #define LSH_HORIZON_DEGREES 1   /* write float32 degrees (contract.py:362-371) instead of slopes */
@@ -383,6 +385,30 @@ int lsh_cbin_decode(const uint8_t* payload, int64_t bytes, int32_t pixels, int32
                     float* degrees, int32_t threads);
 int lsh_cbin_complete(const uint8_t* payload, int64_t bytes, int32_t pixels, int32_t azimuths,
                       int32_t* complete);
+
+/* Horizon tile files (host only). The kind of a file is its extension (.bin
+ * raw little-endian float32 [pixels][azimuths], .cbin compressed).
+ * lsh_tile_file_status: HorizonTileStore.is_complete (file_format.py:335-356):
+ * *status = LSH_TILE_ABSENT (no regular file), LSH_TILE_COMPLETE (a .bin of
+ * exactly the raw size or raw + 28 legacy metadata bytes; a .cbin whose blocks
+ * all parse with nothing after the last) or LSH_TILE_INCOMPLETE.
+ * lsh_tile_file_read: read_horizon_tile_with_timings (file_format.py:219-274):
+ * degrees float32 [pixels][azimuths]; seconds (nullable) double[2] = file read,
+ * decode. LSH_EINVAL names an unreadable file, a wrong raw size or the first
+ * structural error of a .cbin.
+ * lsh_tile_file_write: HorizonTileStore.write's staged publish
+ * (file_format.py:378-415): the raw values or the .cbin payload (compress = 1)
+ * go to a new file beside `path`, are fsynced and renamed over `path`; on any
+ * failure the staging file is removed and LSH_EIO returned. NaN is rejected
+ * (LSH_EINVAL) before anything is written. The directory must exist. */
+#define LSH_TILE_ABSENT 0
+#define LSH_TILE_COMPLETE 1
+#define LSH_TILE_INCOMPLETE 2
+int lsh_tile_file_status(const char* path, int32_t pixels, int32_t azimuths, int32_t* status);
+int lsh_tile_file_read(const char* path, int32_t pixels, int32_t azimuths, float* degrees,
+                       int32_t threads, double* seconds);
+int lsh_tile_file_write(const char* path, const float* degrees, int32_t pixels, int32_t azimuths,
+                        int32_t compress, int32_t threads);
 
 /* Measurement probes (no counterpart in the reference; they calibrate the
  * roofline of the march on the running device).

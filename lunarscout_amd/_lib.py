@@ -22,6 +22,11 @@ LSH_EINVAL = -1
 LSH_ENODEV = -2
 LSH_EDEVICE = -3
 LSH_ENOMEM = -4
+LSH_EIO = -5
+
+TILE_ABSENT = 0
+TILE_COMPLETE = 1
+TILE_INCOMPLETE = 2
 
 HORIZON_DEGREES = 1
 HORIZON_CAP = 2
@@ -111,6 +116,9 @@ SIGNATURES = {
     "lsh_cbin_encode": [_vp, _i32, _i32, _vp, _i64, ctypes.POINTER(_i64), _i32],
     "lsh_cbin_decode": [_vp, _i64, _i32, _i32, _vp, _i32],
     "lsh_cbin_complete": [_vp, _i64, _i32, _i32, ctypes.POINTER(_i32)],
+    "lsh_tile_file_status": [ctypes.c_char_p, _i32, _i32, ctypes.POINTER(_i32)],
+    "lsh_tile_file_read": [ctypes.c_char_p, _i32, _i32, _vp, _i32, _vp],
+    "lsh_tile_file_write": [ctypes.c_char_p, _vp, _i32, _i32, _i32, _i32],
     "lsh_horizon_forward": [_vp, _vp, _i32, _vp, _i32, _i32, _i32, _f32, _i32, _vp, _vp],
     "lsh_sun_loss": [_vp, _i32, _vp, _i32, _vp, _i32, _vp, _i32, _f64, _f64, _vp, _vp],
     "lsh_horizon_adjoint": [_vp, _vp, _i32, _vp, _i32, _i32, _i32, _f32, _vp, _vp, _vp, _f64, _vp],
@@ -165,6 +173,8 @@ def check(status: int, what: str) -> None:
         raise ContractValidationError(message)
     if status == LSH_ENODEV:
         raise HipBackendError(message)
+    if status == LSH_EIO:
+        raise OSError(message)
     raise HipExecutionError(message)
 
 

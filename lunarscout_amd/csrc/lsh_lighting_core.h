@@ -14,6 +14,7 @@
 #include <cstdint>
 
 #include "lsh_internal.h"
+#include "lsh_math.h"
 
 #if defined(__HIPCC__)
 #define LSH_LHD __host__ __device__ inline
@@ -73,16 +74,18 @@ LSH_LHD Frame pixel_frame(const LightArgs& a, int row, int column) {
   return f;
 }
 
-// _azimuth_elevation_deg (psr.py:50-58) in float64.
+// _azimuth_elevation_deg (psr.py:50-58) in float64: az = atan2(e, n) in degrees
+// wrapped into [0, 360) as Python's % does (atan2 lies in (-180, 180], so one
+// addition of 360 to a negative angle is that remainder), el = atan2(u, |(e, n)|).
 LSH_LHD void azimuth_elevation(const Frame& f, const double* v, double& az, double& el) {
   double e[3];
   for (int j = 0; j < 3; ++j)
     e[j] = v[0] * f.rot[0][j] + v[1] * f.rot[1][j] + v[2] * f.rot[2][j] + f.tr[j];
-  double a = atan2(e[0], e[1]) * (180.0 / kPi);
-  a = fmod(a, 360.0);
+  double a = atan2_finite(e[0], e[1]) * (180.0 / kPi);
   if (a < 0.0) a += 360.0;
   az = a;
-  el = atan2(e[2], hypot(e[0], e[1])) * (180.0 / kPi);
+  const double horizontal = __builtin_sqrt(e[0] * e[0] + e[1] * e[1]);
+  el = atan2_finite(e[2], horizontal) * (180.0 / kPi);
 }
 
 template <bool kAzimuthMajor>
@@ -133,6 +136,26 @@ LSH_LHD float sun_fraction(const Row& hz, int N, float az_deg, float el_deg,
   int right = left + 1;
   if (right >= N) right = 0;
   float le = hz[left], re = hz[right];
+  if (w.step <= 0.135f) {
+    // The walk interpolates within bins left .. left + 4 (frac + 15 steps < 4
+    // bins). A disk whose lowest slice top clears the highest of them by 1e-3
+    // deg adds every full slice height (photons = max_photons exactly: doubling
+    // commutes with rounding), one whose highest slice top stays 1e-3 deg under
+    // the lowest adds nothing; rounding of the interpolation is ~1e-5 deg, so
+    // the walk would give the same 1 or 0. Non-finite bins take the walk.
+    int b = right;
+    float lo = le < re ? le : re, hi = le < re ? re : le;
+    for (int k = 0; k < 3; ++k) {
+      b = b + 1 == N ? 0 : b + 1;
+      const float v = hz[b];
+      lo = v < lo ? v : lo;
+      hi = v > hi ? v : hi;
+    }
+    const float c_max = a.half_circle[7] > a.half_circle[8] ? a.half_circle[7] : a.half_circle[8];
+    const bool finite = lo > -1e30f && hi < 1e30f;
+    if (finite && el_deg - c_max > hi + 1e-3f) return 1.0f;
+    if (finite && el_deg + c_max + 1e-3f < lo) return 0.0f;
+  }
   float delta = re - le;
   float photons = 0.0f;
 #pragma unroll
@@ -219,12 +242,16 @@ LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origi
       a.earth_visible[tile_pixels + pixel] = static_cast<float>(visible) / a.earth_count;
   }
   if (a.psr_count > 0 && (a.psr || a.psr_margin)) {
+    // the mask alone is decided by the first vector that lights the limb
+    // (psr.py:136-139); the margin needs every vector
+    const bool mask_only = a.psr_margin == nullptr;
     double best = -1e300;
     for (int t = 0; t < a.psr_count; ++t) {
       double az, el;
       azimuth_elevation(f, a.psr_vectors + 3 * t, az, el);
       double m = el - horizon_at(hz, N, az);
       best = m > best ? m : best;
+      if (mask_only && best > a.psr_threshold) break;
     }
     if (a.psr) a.psr[tile_pixels + pixel] = best > a.psr_threshold ? 0 : 255;
     if (a.psr_margin) a.psr_margin[tile_pixels + pixel] = static_cast<float>(best);

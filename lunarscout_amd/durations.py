@@ -32,6 +32,7 @@ Drop-ins for the reference's landed-mission duration and safe-haven products
 from __future__ import annotations
 
 import bisect
+import re
 from collections.abc import Callable
 from dataclasses import dataclass, field
 from datetime import datetime, timedelta, timezone
@@ -48,18 +49,21 @@ from .geometry import DemGrid
 DurationUnit = Literal["hours", "days"]
 
 
+_ZULU = re.compile(r"Z$")
+
+
 def _utc(value: datetime | str) -> datetime:
-    """mission_duration.py:17-28: aware datetimes or ISO strings with an offset."""
-    if isinstance(value, datetime):
-        parsed = value
-    else:
-        text = str(value).strip()
-        if text.endswith("Z"):
-            text = text[:-1] + "+00:00"
-        parsed = datetime.fromisoformat(text)
-    if parsed.tzinfo is None or parsed.utcoffset() is None:
+    """A timestamp as an aware UTC datetime (the rule of mission_duration.py:17-28).
+
+    Accepts an aware datetime or an ISO-8601 string carrying its offset (a
+    trailing ``Z`` is UTC); a naive timestamp is rejected.
+    """
+    stamp = (value if isinstance(value, datetime)
+             else datetime.fromisoformat(_ZULU.sub("+00:00", str(value).strip())))
+    offset = stamp.utcoffset()
+    if offset is None:
         raise ValueError("mission-duration timestamps must include a UTC offset")
-    return parsed.astimezone(timezone.utc)
+    return (stamp - offset).replace(tzinfo=timezone.utc)
 
 
 @dataclass(frozen=True)

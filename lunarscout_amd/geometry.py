@@ -57,6 +57,29 @@ class ProjectionParameters:
         )
 
 
+PROJECTION_PARAMETER_FIELDS = (
+    "radius_m", "latitude_origin_rad", "longitude_origin_rad", "scale",
+    "false_easting_m", "false_northing_m",
+)
+
+
+def projection_array(projection) -> npt.NDArray[np.float64]:
+    """float64[6] of a projection in PROJECTION_PARAMETER_FIELDS order (contract.py:28-31).
+
+    Reads the named fields, so the reference's own ``ProjectionParameters``
+    (geometry.py:24-38, which has no array method) is accepted as well as this
+    module's; a sequence of six numbers is taken as already in that order.
+    """
+    if all(hasattr(projection, f) for f in PROJECTION_PARAMETER_FIELDS):
+        values = [getattr(projection, f) for f in PROJECTION_PARAMETER_FIELDS]
+    else:
+        values = list(np.asarray(projection, dtype=np.float64).reshape(-1))
+    array = np.asarray(values, dtype=np.float64)
+    if array.shape != (6,) or not np.all(np.isfinite(array)):
+        raise ContractValidationError("projection parameters must be six finite values")
+    return array
+
+
 @dataclass(frozen=True, slots=True)
 class DemGrid:
     """float32 elevation raster with an affine transform (geometry.py:41-116)."""

@@ -25,7 +25,7 @@ import numpy as np
 from . import _lib
 from ._lib import DeviceBuffer, check
 from .errors import ContractValidationError
-from .geometry import DemGrid
+from .geometry import DemGrid, ProjectionParameters, projection_array
 
 PATCH_SIZE = 128
 AZIMUTH_COUNT = 1440
@@ -44,6 +44,40 @@ class PyramidArrays:
     @property
     def level0(self) -> np.ndarray:
         return self.dem.elevation_m
+
+
+def dem_of_pyramid(pyramid) -> tuple[DemGrid, np.ndarray]:
+    """The DEM a pyramid describes and its concatenated mips.
+
+    Takes this module's ``PyramidArrays`` or the reference contract's
+    (contract.py:274-281): ``level0`` float32 [H, W], ``mips`` float32, ``levels``
+    int32 [L, 4], ``map_parameters`` float32[11] (radius, scale, false
+    easting/northing, 1 / determinant, the six transform coefficients;
+    pyramid.py:78-107) and ``projection_parameters`` float32[6]. The march reads
+    the map resolution and radius in float32 from those values, so rebuilding a
+    float64 transform from the float32 parameters changes nothing it computes;
+    the level table must be this pyramid layout's.
+    """
+    if hasattr(pyramid, "dem"):
+        return pyramid.dem, np.ascontiguousarray(pyramid.mips, dtype=np.float32)
+    try:
+        level0 = np.ascontiguousarray(pyramid.level0, dtype=np.float32)
+        map_parameters = np.asarray(pyramid.map_parameters, dtype=np.float64).reshape(-1)
+        projection = np.asarray(pyramid.projection_parameters, dtype=np.float64).reshape(-1)
+        levels = np.asarray(pyramid.levels, dtype=np.int32)
+        mips = np.ascontiguousarray(pyramid.mips, dtype=np.float32)
+    except AttributeError as exc:
+        raise ContractValidationError(
+            "pyramids must carry level0, mips, levels, map_parameters and "
+            "projection_parameters") from exc
+    if level0.ndim != 2 or map_parameters.shape != (11,):
+        raise ContractValidationError("pyramid level 0 must be 2-D with 11 map parameters")
+    dem = DemGrid(level0, np.ascontiguousarray(map_parameters[5:11]),
+                  ProjectionParameters.from_array(projection))
+    expected, _ = pyramid_layout(dem.width, dem.height)
+    if levels.shape != expected.shape or not np.array_equal(levels, expected):
+        raise ContractValidationError("pyramid level metadata does not match the DEM's layout")
+    return dem, mips
 
 
 def pyramid_layout(width: int, height: int) -> tuple[np.ndarray, int]:
@@ -71,6 +105,7 @@ class HipSession(_lib.Session):
         self._dem_lock = threading.Lock()
         self._segments: DeviceBuffer | None = None
         self._output: DeviceBuffer | None = None
+        self._pyramid_dems: dict[int, tuple[object, DemGrid]] = {}
         self.last_production_timings: dict | None = None
 
     def close(self) -> None:
@@ -81,6 +116,7 @@ class HipSession(_lib.Session):
                 buf.free()
                 setattr(self, name, None)
         self._dems = {}
+        self._pyramid_dems = {}
         super().close()
 
     # -- DEM residency ---------------------------------------------------
@@ -92,7 +128,7 @@ class HipSession(_lib.Session):
                 return entry.dem_id
             elevation = np.ascontiguousarray(dem.elevation_m, dtype=np.float32)
             gt = np.ascontiguousarray(dem.geo_transform, dtype=np.float64)
-            proj = dem.projection.as_array()
+            proj = projection_array(dem.projection)
             mips_ptr = None
             if mips is not None:
                 _, expected = pyramid_layout(dem.width, dem.height)
@@ -188,6 +224,8 @@ class HipSession(_lib.Session):
     ) -> tuple[np.ndarray, list[float]]:
         """Every DEM pass on one resident slope buffer (cuda_backend.py:1445-1552).
 
+        ``pyramids`` are this module's ``PyramidArrays`` or the reference
+        contract's (see ``dem_of_pyramid``), one per DEM pass, primary first.
         Returns float32 slopes [tile_width * tile_height, N] and the kernel seconds
         (all passes run in one launch, so the list holds one entry).
         """
@@ -198,9 +236,15 @@ class HipSession(_lib.Session):
             raise ValueError("pyramid count must match the segment DEM axis")
         if segments.shape[1] != (tile_width // subpatch_size + 2) ** 2:
             raise ValueError("segment center count does not match tile/subpatch layout")
-        dems = [p.dem for p in pyramids]
+        dems = []
         for p in pyramids:
-            self.dem_id(p.dem, p.mips)
+            # one DemGrid per pyramid object, kept with it, so its device copy is reused
+            held = self._pyramid_dems.get(id(p))
+            if held is None or held[0] is not p:
+                dem, mips = dem_of_pyramid(p)
+                self.dem_id(dem, mips)
+                held = self._pyramid_dems[id(p)] = (p, dem)
+            dems.append(held[1])
         started = time.perf_counter()
         seg_buf = DeviceBuffer.from_host(self, segments)
         upload = time.perf_counter() - started

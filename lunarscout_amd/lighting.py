@@ -23,7 +23,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import DeviceBuffer, check
-from .geometry import DemGrid
+from .geometry import DemGrid, projection_array
 
 PATCH_SIZE = 128
 PATCH_PIXELS = PATCH_SIZE * PATCH_SIZE
@@ -46,7 +46,7 @@ def reduce_sun_vectors_for_psr(dem: DemGrid, sun_vectors_m, azimuth_count: int =
     lib = _lib.load_library()
     elevation = np.ascontiguousarray(dem.elevation_m, dtype=np.float32)
     gt = np.ascontiguousarray(dem.geo_transform, dtype=np.float64)
-    proj = dem.projection.as_array()
+    proj = projection_array(dem.projection)
     indices = np.empty(5 * azimuth_count, dtype=np.int64)
     kept = ctypes.c_int32()
     check(lib.lsh_psr_reduce(elevation.ctypes.data, dem.width, dem.height, gt.ctypes.data,
@@ -357,7 +357,7 @@ class CpuLightingSession:
                 ptrs[name] = None
         elevation = np.ascontiguousarray(dem.elevation_m, dtype=np.float32)
         gt = np.ascontiguousarray(dem.geo_transform, dtype=np.float64)
-        proj = dem.projection.as_array()
+        proj = projection_array(dem.projection)
 
         def ptr(v):
             return None if v is None else v.ctypes.data
@@ -407,7 +407,7 @@ class CpuLightingSession:
         out = np.empty((plan.bands, n, PATCH_PIXELS), np.float32)
         elevation = np.ascontiguousarray(dem.elevation_m, dtype=np.float32)
         gt = np.ascontiguousarray(dem.geo_transform, dtype=np.float64)
-        proj = dem.projection.as_array()
+        proj = projection_array(dem.projection)
         check(self.lib.lsh_duration_tiles_host(
             elevation.ctypes.data, dem.width, dem.height, gt.ctypes.data, proj.ctypes.data,
             tile_arr.ctypes.data, n, hz.ctypes.data, N, 1 if azimuth_major else 0,

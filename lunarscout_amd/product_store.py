@@ -30,6 +30,7 @@ from typing import Any
 import numpy as np
 
 from .errors import ProductGenerationError
+from .geometry import projection_array
 from .geotiff import GeoTiffWriter
 
 MANIFEST_SCHEMA = "lunarscout-amd-product-job-v1"
@@ -82,7 +83,7 @@ class ResumableTiledProduct:
             "dtype": np.dtype(dtype).str, "nodata": _nodata_json(nodata),
             "compression": "deflate" if compress else "none",
             "affine_transform": [float(v) for v in affine_transform],
-            "projection": [float(v) for v in projection.as_array()],
+            "projection": [float(v) for v in projection_array(projection)],
             "descriptions": list(descriptions or []),
             "configuration": configuration or {},
             "tile_size": PATCH_SIZE, "interleave": "band",

@@ -87,3 +87,13 @@ def trace_ray(pyramids, segment_values, *, tile_column, tile_row, pixel, azimuth
                              ctypes.c_int32(azimuth), v(rows.ctypes.data),
                              ctypes.c_int32(capacity), ctypes.byref(res))
     return res.value, rows[:min(n, capacity)].copy()
+
+
+def atan2(y, x):
+    """The lighting path's float64 arctangent (lsh_math.h) of arrays y, x."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(y)
+    v = ctypes.c_void_p
+    lib().kcpu_atan2(v(y.ctypes.data), v(x.ctypes.data), ctypes.c_int64(y.size), v(out.ctypes.data))
+    return out

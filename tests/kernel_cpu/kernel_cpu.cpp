@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../lunarscout_amd/csrc/lsh_march_core.h"
+#include "../../lunarscout_amd/csrc/lsh_math.h"
 
 namespace {
 struct PassTables {
@@ -233,6 +234,11 @@ int kcpu_trace_ray(const kc_pyramid* pyrs, int32_t D, const float* segments, int
                      pixel / tile_width, tile_column, tile_row, tile_width, subpatch, obs_h, t,
                      &best, cnt, tr);
   return n;
+}
+
+// The lighting path's float64 arctangent over n (y, x) pairs.
+void kcpu_atan2(const double* y, const double* x, int64_t n, double* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = lsh::atan2_finite(y[i], x[i]);
 }
 
 // hierarchy_maximum over one raw 18-field segment (no subpatch blend), the

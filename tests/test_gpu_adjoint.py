@@ -1,19 +1,16 @@
 """The adjoint: dL/dDEM of the smoothed sun-fraction loss.
 
 * the loss and dL/d(horizon) against a pure-PyTorch autograd reference of the
-  same smoothed 16-slice model (float64);
-* dL/dDEM against central finite differences of the float64 oracle under the
-  adjoint's convention -- sample positions and culls held at the forward's (the
-  march's adaptive steps also move samples continuously with the heights; the
-  winning-sample adjoint does not differentiate that): for each probed node the
-  affected pixels (those whose winning sample touches the node, and the pixel
-  the node carries) are re-evaluated by the oracle with the node moved by +-h,
-  and their loss recomputed in float64. The assertion uses h = 1 mm (tie-free:
-  h and h/2 agree to 1e-4) and requires <= 1e-3 relative error on at least 30
-  nodes (24 primary, 16 regional candidates). At h = 5 cm
-  near-field rays move their horizon by up to ~0.3 deg, more than the loss's
-  0.02-deg disk-edge rounding, so those differences are secants across the disk
-  edge; they are printed, not asserted;
+  same smoothed 16-slice model (float64), on a small two-DEM case;
+* dL/dDEM of the adjoint configuration (512^2 @10 m + 1024^2 @60 m, 16 tiles x
+  720 azimuths, 2190 Sun vectors) against central finite differences with
+  h = 5 cm of the float64 oracle under the adjoint's convention -- sample
+  positions and culls held at the forward's (the march's adaptive steps also
+  move samples continuously with the heights; the winning-sample adjoint does
+  not differentiate that): for each probed node the affected pixels (those
+  whose winning sample touches the node, and the pixel the node carries) are
+  re-evaluated by the oracle with the node moved by +-h, and their loss
+  recomputed in float64; at least 200 tie-free nodes within 1e-3 relative;
 * run-to-run bit-identical gradients (fixed-point scatter), and the
   torch.autograd.Function wrapper.
 """
@@ -21,6 +18,7 @@
 from __future__ import annotations
 
 import functools
+import time
 
 import numpy as np
 import pytest
@@ -109,21 +107,33 @@ def _perturbed(dem: DemGrid, x: int, y: int, delta: float) -> tuple[DemGrid, flo
                                                                np.float64(dem.elevation_m[y, x]))
 
 
-def _affected(problem, cells, dem_index, x, y):
-    """(tile, pixel) pairs whose loss can change when node (x, y) of the DEM moves."""
-    out = set()
+def _affected_pixels(problem, cells, nodes):
+    """{(dem, x, y): sorted (tile, pixel) pairs whose loss can move with that node}.
+
+    A pixel is affected when the winning sample of one of its rays interpolates
+    the node (the sample's cell is one of the four cells around it) or, for the
+    primary DEM, when the node is the pixel's own observer terrain.
+    """
+    code_to_nodes: dict[int, list] = {}
+    for node in nodes:
+        d, x, y = node
+        for cy in (y - 1, y):
+            for cx in (x - 1, x):
+                code_to_nodes.setdefault((d << 28) | (cy << 14) | cx, []).append(node)
+    codes = np.fromiter(code_to_nodes.keys(), dtype=np.int64)
+    out = {node: set() for node in nodes}
     for t, (tx, ty) in enumerate(problem.tiles):
-        c = cells[t]
-        valid = c >= 0
-        passes = np.where(valid, c >> 28, -1)
-        cy = (c >> 14) & 0x3FFF
-        cx = c & 0x3FFF
-        hit = valid & (passes == dem_index) & ((cx == x) | (cx == x - 1)) & ((cy == y) | (cy == y - 1))
-        for p in np.nonzero(hit.any(axis=1))[0]:
-            out.add((t, int(p)))
-        if dem_index == 0 and tx <= x < tx + 128 and ty <= y < ty + 128:
-            out.add((t, (y - ty) * 128 + (x - tx)))
-    return sorted(out)
+        c = cells[t].astype(np.int64)
+        hit = np.isin(c, codes)
+        pix, az = np.nonzero(hit)
+        for p, code in set(zip(pix.tolist(), c[pix, az].tolist())):
+            for node in code_to_nodes[code]:
+                out[node].add((t, p))
+        for node in nodes:
+            d, x, y = node
+            if d == 0 and tx <= x < tx + 128 and ty <= y < ty + 128:
+                out[node].add((t, (y - ty) * 128 + (x - tx)))
+    return {node: sorted(v) for node, v in out.items()}
 
 
 def _fd_loss(problem, dems, affected, segments, pyramids):
@@ -144,56 +154,110 @@ def _fd_loss(problem, dems, affected, segments, pyramids):
     return total
 
 
-def test_adjoint_matches_finite_differences_of_the_float64_oracle():
-    problem, _, grads, cells = _case()
+# The adjoint configuration: 512^2 @10 m (seed 7) + 1024^2 @60 m (seed 8), 16 tiles
+# x 720 azimuths, 2190 Sun vectors at 4-hour steps over 365 days (seed 10).
+FD_STEP_M = 0.05
+FD_SMOOTHING_DEG = 0.08
+FD_NODES = 200
+FD_MAX_AFFECTED = 24
+
+
+@functools.lru_cache(maxsize=None)
+def _config5():
+    from lunarscout_amd.adjoint import SunLossProblem
+
+    dems = synthetic.site_dems(synthetic.ADJOINT_SITE)
+    tiles = [(x, y) for y in range(0, 512, 128) for x in range(0, 512, 128)]
+    sun = synthetic.sun_vectors(dems[0], 2190, 4.0, seed=10)
+    problem = SunLossProblem(dems, tiles, sun, azimuth_count=N, smoothing_deg=FD_SMOOTHING_DEG)
+    loss = problem.forward()
+    grads, cells = problem.backward(sample_cells=True)
+    return problem, loss, grads, cells
+
+
+def test_adjoint_matches_finite_differences_at_5_cm_on_200_nodes():
+    """dL/dDEM of the config-5 problem against central differences with h = 5 cm.
+
+    The loss's slice clamps are rounded over +-0.08 deg (at most the narrowest
+    slice half-height, 0.094 deg): a 5 cm move of a node changes the horizon of
+    a ray that samples it r metres away by about 2.9 / r degrees, so rays beyond
+    a few hundred metres stay inside the rounding and 5 cm is a linear step
+    there. A node is used when it is tie-free and linear at that step: its
+    forward and backward differences at h = 5 cm agree to 2e-4 of the central
+    one (curvature over the step, a winner switch or a slice edge crossed inside
+    it, or a tie at the base state -- a kink that central differences alone
+    average away -- breaks that). Nodes
+    are drawn at random among interior nodes of both DEMs whose gradient is
+    at least 1e5 fixed-point steps of the backward's scatter, with at most 24
+    affected pixels each (so the oracle finishes in seconds); at least 200 must
+    pass and agree with the adjoint to 1e-3 relative.
+    """
+    problem, _, grads, cells = _config5()
     dems = problem.dems
+    t0 = time.time()
     segments = [oracle.segments(dems, tile_column=tx, tile_row=ty, tile_width=128, azimuth_count=N)
                 for tx, ty in problem.tiles]
-    norm = problem.valid_pixels * len(problem.sun)
     pyramids = [oracle.OraclePyramid(d) for d in dems]
-    rng = np.random.default_rng(1)
-    checked = rejected = 0
-    worst = 0.0
-    coarse = []
-    for dem_index, count in ((0, 24), (1, 16)):
-        done = 0
+    norm = problem.valid_pixels * len(problem.sun)
+    rng = np.random.default_rng(5)
+    candidates = []
+    for dem_index, count in ((0, 2000), (1, 1500)):
         g = grads[dem_index]
         h_, w_ = g.shape
         interior = np.zeros_like(g, dtype=bool)
         interior[1:h_ - 2, 1:w_ - 2] = True
-        magnitude = np.abs(g) * interior
-        candidates = np.argwhere(magnitude > np.quantile(magnitude[magnitude > 0], 0.5))
-        for y, x in candidates[rng.choice(len(candidates), min(5 * count, len(candidates)), replace=False)]:
-            if done >= count:
-                break
-            affected = _affected(problem, cells, dem_index, int(x), int(y))
-            if not affected or len(affected) > 120:
-                continue
-            fd = []
-            for step in (1e-3, 5e-4, 0.05):
-                values, moves = [], []
-                for sign in (1.0, -1.0):
-                    moved = list(dems)
-                    moved[dem_index], actual = _perturbed(dems[dem_index], int(x), int(y), sign * step)
-                    values.append(_fd_loss(problem, moved, affected, segments, pyramids))
-                    moves.append(actual)
-                fd.append((values[0] - values[1]) / (moves[0] - moves[1]) / norm)
-            if abs(fd[0] - fd[1]) > 1e-4 * max(abs(fd[0]), 1e-30):
-                rejected += 1  # a winner switches inside the step
-                continue
-            rel = abs(g[y, x] - fd[0]) / max(abs(fd[0]), 1e-30)
-            rel_coarse = abs(g[y, x] - fd[2]) / max(abs(fd[2]), 1e-30)
-            coarse.append(rel_coarse)
-            print(f"DEM {dem_index} node ({x},{y}): adjoint {g[y, x]:.6e} fd(1 mm) {fd[0]:.6e} "
-                  f"rel {rel:.2e}; fd(5 cm) {fd[2]:.6e} rel {rel_coarse:.2e} ({len(affected)} px)",
-                  flush=True)
-            worst = max(worst, rel)
-            checked += 1
-            done += 1
-            assert rel <= 1e-3, (dem_index, x, y)
-    print(f"checked {checked} tie-free nodes, rejected {rejected}, worst rel {worst:.2e}; "
-          f"at h = 5 cm within 1e-3: {np.mean(np.array(coarse) <= 1e-3):.2f}", flush=True)
-    assert checked >= 30
+        # the backward sums contributions in 64-bit fixed point with steps of
+        # 1 / fixed_scale of the un-normalised sum: a node is compared only where
+        # its gradient is at least 1e5 such steps (quantisation below 1e-4
+        # relative even with tens of contributions)
+        units = np.abs(g) * norm * problem.fixed_scale
+        nz = np.argwhere((units >= 1e5) & interior)
+        pick = nz[rng.choice(len(nz), min(count, len(nz)), replace=False)]
+        candidates += [(dem_index, int(x), int(y)) for y, x in pick]
+    order = rng.permutation(len(candidates))
+    candidates = [candidates[i] for i in order]
+    affected = _affected_pixels(problem, cells, candidates)
+    print(f"config 5: {len(candidates)} candidate nodes, affected sets in {time.time() - t0:.1f} s",
+          flush=True)
+    accepted = {0: 0, 1: 0}
+    rejected_nonlinear = skipped = 0
+    worst = 0.0
+    for node in candidates:
+        if sum(accepted.values()) >= FD_NODES:
+            break
+        d, x, y = node
+        pixels = affected[node]
+        if not pixels or len(pixels) > FD_MAX_AFFECTED:
+            skipped += 1
+            continue
+        values, moves = [], []
+        for sign in (1.0, 0.0, -1.0):
+            moved = list(dems)
+            moved[d], actual = _perturbed(dems[d], x, y, sign * FD_STEP_M)
+            values.append(_fd_loss(problem, moved, pixels, segments, pyramids))
+            moves.append(actual)
+        central = (values[0] - values[2]) / (moves[0] - moves[2]) / norm
+        forward = (values[0] - values[1]) / (moves[0] - moves[1]) / norm
+        backward = (values[1] - values[2]) / (moves[1] - moves[2]) / norm
+        # one-sided differences that disagree mean curvature over the step or a
+        # kink at the base (a winner tie): not tie-free and linear at 5 cm
+        if abs(forward - backward) > 2e-4 * max(abs(central), 1e-30):
+            rejected_nonlinear += 1
+            continue
+        fd = [central]
+        rel = abs(grads[d][y, x] - fd[0]) / max(abs(fd[0]), 1e-30)
+        worst = max(worst, rel)
+        accepted[d] += 1
+        if sum(accepted.values()) % 25 == 0:
+            print(f"  {sum(accepted.values())} nodes checked, worst rel {worst:.2e} "
+                  f"({time.time() - t0:.0f} s)", flush=True)
+        assert rel <= 1e-3, (node, grads[d][y, x], fd[0])
+    print(f"config 5: {sum(accepted.values())} nodes within 1e-3 of central differences at "
+          f"h = 5 cm (primary {accepted[0]}, regional {accepted[1]}), worst rel {worst:.2e}; "
+          f"rejected as non-linear or tied at 5 cm: {rejected_nonlinear}; skipped (more than "
+          f"{FD_MAX_AFFECTED} affected px): {skipped}; {time.time() - t0:.0f} s", flush=True)
+    assert sum(accepted.values()) >= FD_NODES
+    assert accepted[0] > 0 and accepted[1] > 0
 
 
 def test_gradients_are_bit_identical_run_to_run():

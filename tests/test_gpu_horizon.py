@@ -198,3 +198,44 @@ def test_captured_production_buffer_with_the_diagnostic_cap():
     expected = fx.rays_arrays()[f"{base}__final_slopes"][0]
     others = np.arange(1440) != 360
     np.testing.assert_allclose(out[others], expected[others], rtol=0, atol=2e-5)
+
+
+def test_session_seam_takes_reference_pyramid_arrays_as_the_generator_calls_it():
+    """generate_patch_horizons (generator.py:27-36) with the reference's PyramidArrays.
+
+    The seam gets the captured subpatch segment tensor and pyramid look-alikes
+    built from the fixture arrays PyramidArrays.from_artifact reads; its slopes
+    must equal the plain march of the same segments over the fixture DEMs bit
+    for bit, and the oracle within the parity bar.
+    """
+    from lunarscout_amd._lib import DeviceBuffer
+    from tests.test_seams import reference_pyramid
+
+    meta = fx.rays_metadata()
+    fixture = next(f for f in meta["subpatch_fixtures"] if f["id"] == "boundary_halo_multi_dem_16az")
+    cfg = fixture["configuration"]
+    prefixes = ["multi_dem_outer_obstacle_east__dem_0", "multi_dem_outer_obstacle_east__dem_1"]
+    values = np.ascontiguousarray(
+        fx.rays_arrays()[f"{fixture['id']}__subpatch_fixture__segment_values"], dtype=np.float32)
+    s = session()
+    slopes, seconds = s.subpatch_hierarchical_all_passes(
+        values, [reference_pyramid(p) for p in prefixes], tile_column=cfg["tile_column"],
+        tile_row=cfg["tile_row"], tile_width=cfg["tile_width"], tile_height=cfg["tile_height"],
+        subpatch_size=cfg["subpatch_size"], observer_elevation_m=cfg["observer_elevation_m"])
+    assert slopes.shape == (cfg["tile_width"] * cfg["tile_height"], cfg["azimuth_count"])
+    assert len(seconds) == 1
+    dems = [fx.dem(p) for p in prefixes]
+    seg = DeviceBuffer.from_host(s, values)
+    plain = s.march_device(dems, seg, tile_column=cfg["tile_column"], tile_row=cfg["tile_row"],
+                           tile_width=cfg["tile_width"], tile_height=cfg["tile_height"],
+                           subpatch_size=cfg["subpatch_size"],
+                           observer_elevation_m=cfg["observer_elevation_m"]).to_host()
+    seg.free()
+    same = (plain == slopes) | (np.isneginf(plain) & np.isneginf(slopes))
+    assert same.all()
+    ref, _ = oracle.horizon_tile([oracle.OraclePyramid(d) for d in dems], values,
+                                 tile_column=cfg["tile_column"], tile_row=cfg["tile_row"],
+                                 tile_width=cfg["tile_width"], tile_height=cfg["tile_height"],
+                                 subpatch_size=cfg["subpatch_size"],
+                                 observer_elevation_m=cfg["observer_elevation_m"], float64=True)
+    assert_horizon_parity(slopes, ref, "seam with reference PyramidArrays")

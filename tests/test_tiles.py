@@ -126,7 +126,7 @@ def test_store_writes_pads_edges_and_reads_back(tmp_path):
     for compress in (True, False):
         path = store.write(0, 128, 0.0, valid, compress=compress, valid_width=77, valid_height=100)
         assert path.is_file() and store.is_complete(path)
-        assert not [p for p in path.parent.iterdir() if ".tmp" in p.name]
+        assert [p.name for p in path.parent.iterdir() if p.name.startswith(path.name)] == [path.name]
         back = store.read(0, 128, 0.0) if compress else tiles.read_horizon_tile(path)
         assert back.shape == (128, 128, 1440)
         assert np.all(back[100:] == np.float32(-50.0)) and np.all(back[:, 77:] == np.float32(-50.0))
@@ -152,3 +152,40 @@ def test_read_into_output_buffer(tmp_path):
     with pytest.raises(ValueError):
         tiles.read_horizon_tile_with_timings(path, output=np.empty((128, 128, 1440), np.float64))
     os.remove(path)
+
+
+def test_tile_files_status_read_and_failed_publish(tmp_path):
+    from lunarscout_amd import _lib
+
+    rng = np.random.default_rng(9)
+    deg = rng.uniform(-20, 30, (16384, 1440)).astype(np.float32)
+    path = tiles.write_tile_file(tmp_path / "a" / "horizon_00000_00000_000.cbin", deg, compress=True)
+    assert tiles.tile_file_status(path) == _lib.TILE_COMPLETE
+    payload = path.read_bytes()
+    # a torn write (truncated) or trailing garbage is present but not complete
+    torn = tmp_path / "torn.cbin"
+    torn.write_bytes(payload[:-3])
+    assert tiles.tile_file_status(torn) == _lib.TILE_INCOMPLETE
+    with pytest.raises(ValueError, match="ended"):
+        tiles.read_horizon_tile(torn)
+    torn.write_bytes(payload + b"\x01")
+    assert tiles.tile_file_status(torn) == _lib.TILE_INCOMPLETE
+    assert tiles.tile_file_status(tmp_path / "missing.cbin") == _lib.TILE_ABSENT
+    # a legacy raw tile carries 28 metadata bytes after the values; readers ignore them
+    legacy = tmp_path / "legacy.BIN"
+    legacy.write_bytes(deg.tobytes() + b"\x7f" * 28)
+    assert tiles.tile_file_status(legacy) == _lib.TILE_COMPLETE
+    np.testing.assert_array_equal(tiles.read_horizon_tile(legacy).reshape(16384, 1440), deg)
+    other = tmp_path / "tile.raw"
+    other.write_bytes(deg.tobytes())
+    assert tiles.tile_file_status(other) == _lib.TILE_INCOMPLETE
+    with pytest.raises(ValueError, match="extension"):
+        tiles.read_horizon_tile(other)
+    # a publish that cannot create its staging file raises OSError and leaves nothing
+    lib = _lib.load_library()
+    target = str(tmp_path / "no_such_dir" / "horizon_00000_00000_000.bin").encode()
+    assert lib.lsh_tile_file_write(target, deg.ctypes.data, 16384, 1440, 0, 0) == _lib.LSH_EIO
+    with pytest.raises(OSError):
+        _lib.check(_lib.LSH_EIO, "write")
+    assert not (tmp_path / "no_such_dir").exists()
+    assert sorted(p.name for p in (tmp_path / "a").iterdir()) == [path.name]
