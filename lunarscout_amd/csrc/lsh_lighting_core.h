@@ -136,9 +136,11 @@ LSH_LHD float sun_fraction(const Row& hz, int N, float az_deg, float el_deg,
   int right = left + 1;
   if (right >= N) right = 0;
   float le = hz[left], re = hz[right];
-  if (w.step <= 0.135f) {
-    // The walk interpolates within bins left .. left + 4 (frac + 15 steps < 4
-    // bins). A disk whose lowest slice top clears the highest of them by 1e-3
+  if (w.step <= 0.135f && frac >= 0.0f) {
+    // With frac >= 0 the walk interpolates within bins left .. left + 4
+    // (frac + 15 steps < 4 bins); a truncated negative position (az below half
+    // the disk plus half a bin) extrapolates left of bin 0 instead and takes the
+    // walk. A disk whose lowest slice top clears the highest of them by 1e-3
     // deg adds every full slice height (photons = max_photons exactly: doubling
     // commutes with rounding), one whose highest slice top stays 1e-3 deg under
     // the lowest adds nothing; rounding of the interpolation is ~1e-5 deg, so

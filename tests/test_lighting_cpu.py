@@ -104,3 +104,40 @@ def test_backend_selection():
 
         with pytest.raises(HipBackendError):
             select_lighting_session("hip")
+
+
+def test_disk_walk_equals_the_reference_walk_bit_for_bit_on_steep_rows():
+    """The product's slice walk (with its whole-disk lit / dark exits) against the
+    oracle's restatement of _sun_fraction_reference (lightmap.py:32-73) in float32.
+
+    Rows are random walks with up to +-4 degrees between bins, each pixel its own
+    row; the Sun sits at float32-exact azimuths (including the truncated negative
+    positions below 0.395 deg and the wrap past 359.6 deg) and at elevations
+    within a disk of the local horizon, fully above and fully below it. The
+    fractions must be identical.
+    """
+    rng = np.random.default_rng(21)
+    dem = DemGrid(np.zeros((128, 128), np.float32),
+                  np.asarray((0.0, 1.0, 0.0, 15162.0 + 64.0, 0.0, -1.0), np.float64),
+                  ProjectionParameters.south_polar())
+    steps = rng.uniform(-4.0, 4.0, (128, 128, 1440)).astype(np.float32)
+    rows = np.cumsum(steps, axis=-1, dtype=np.float32)
+    rows = (rows - rows.mean(axis=-1, keepdims=True)).astype(np.float32)
+    az = np.concatenate([rng.uniform(0.0, 0.4, 40), rng.uniform(359.5, 360.0, 40),
+                         rng.uniform(0.0, 360.0, 120)]).astype(np.float32)
+    pixel = (rng.integers(0, 128), rng.integers(0, 128))
+    row = rows[pixel]
+    bins = (az.astype(np.float64) * 4).astype(int) % 1440
+    el = (row[bins].astype(np.float64) + rng.choice([-6.0, -1.0, -0.2, 0.0, 0.1, 0.3, 1.0, 6.0], az.size)
+          + rng.uniform(-0.1, 0.1, az.size)).astype(np.float32)
+    vectors = np.stack([local_direction_vector(dem, pixel[0], pixel[1], float(a), float(e), 1.5e11)
+                        for a, e in zip(az, el)])
+    got_az, got_el = oracle.azimuth_elevation(dem, pixel[0], pixel[1], vectors)
+    assert np.array_equal(got_az.astype(np.float32), az) and np.array_equal(got_el.astype(np.float32), el)
+    cpu = CpuLightingSession(threads=2)
+    fractions = np.stack(list(cpu.iter_patch_fraction_tiles(dem, rows, vectors, tile_y=0, tile_x=0)))
+    got = fractions[:, pixel[0], pixel[1]]
+    want = np.array([oracle.sun_fraction(row, float(a), float(e), float32=True)
+                     for a, e in zip(az, el)], dtype=np.float32)
+    assert np.count_nonzero(want == 1.0) > 10 and np.count_nonzero(want == 0.0) > 10
+    np.testing.assert_array_equal(got, want)
