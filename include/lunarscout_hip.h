@@ -270,6 +270,25 @@ int lsh_lighting_tiles(lsh_session* session, int32_t dem_id, const lsh_tile* til
                        float* earth_margin, float* earth_visible, uint8_t* psr,
                        float* psr_margin);
 
+/* lsh_lighting_tiles without allocation. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Arguments,
+ * outputs and results are those of lsh_lighting_tiles; the device copy of the
+ * tile list goes to a static table of the library instead of session scratch,
+ * so the call performs no hipMalloc / hipFree. tile_count must be at most 4096
+ * (LSH_EINVAL beyond). The table is one per device, shared by every call on
+ * that device: calls on one stream are ordered, calls that overlap on one
+ * device from different streams or sessions race. */
+int lsh_lighting_tiles_static(lsh_session* session, int32_t dem_id, const lsh_tile* tiles,
+                              int32_t tile_count, const float* horizons, int32_t azimuth_count,
+                              int32_t azimuth_major, const double* sun, int32_t sun_count,
+                              const double* earth, int32_t earth_count,
+                              const double* psr_vectors, int32_t psr_count,
+                              double sun_half_angle_deg, double psr_diameter_deg, int32_t flags,
+                              uint8_t* sun_bytes, float* sun_fraction, float* sun_mean,
+                              float* earth_margin, float* earth_visible, uint8_t* psr,
+                              float* psr_margin);
+
 /* The same lighting on host threads: the "cpu" backend of the lighting products
  * (LightmapCpuSession, lightmap_cpu.py:193-237; compute_psr_patch_reference,
  * psr.py:94-140). Every buffer is host memory: elevation float32

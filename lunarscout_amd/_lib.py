@@ -104,6 +104,8 @@ SIGNATURES = {
     "lsh_horizon_tile_full": [_vp, _vp, _i32, _i32, _i32, _i32, _f64, _f32, _i32, _vp, _vp],
     "lsh_lighting_tiles": [_vp, _i32, _vp, _i32, _vp, _i32, _i32, _vp, _i32, _vp, _i32, _vp, _i32,
                            _f64, _f64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "lsh_lighting_tiles_static": [_vp, _i32, _vp, _i32, _vp, _i32, _i32, _vp, _i32, _vp, _i32,
+                                  _vp, _i32, _f64, _f64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "lsh_lighting_tiles_host": [_vp, _i32, _i32, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _vp, _i32,
                                 _vp, _i32, _vp, _i32, _f64, _f64, _i32, _i32, _vp, _vp, _vp, _vp,
                                 _vp, _vp, _vp],

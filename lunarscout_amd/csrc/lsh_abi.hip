@@ -651,6 +651,77 @@ int lsh_lighting_tiles(lsh_session* session, int32_t dem_id, const lsh_tile* til
   return LSH_OK;
 }
 
+// -- lighting without allocation ---------------------------------------------------
+// The only device memory lsh_lighting_tiles uses beyond the caller's inputs and
+// outputs is the device copy of the tile list (session scratch, grown with
+// hipMalloc). lsh_lighting_tiles_static keeps that copy in this file-scope array
+// instead: 4096 tiles (32 KiB) per device, held until the process ends. Host
+// code never passes `scratch` itself to a kernel: its device address on each
+// device comes from hipGetSymbolAddress, taken once per device with that device
+// current, and only that address is used.
+namespace {
+
+constexpr int32_t kStaticTileCapacity = 4096;
+static __device__ __align__(256) lsh_tile scratch[kStaticTileCapacity];
+
+constexpr int kStaticDevices = 64;
+std::mutex g_scratch_mutex;
+lsh_tile* g_scratch_address[kStaticDevices] = {};
+
+// The device address of `scratch` on the current device (set up on first use).
+int static_tile_table(int device, lsh_tile** out) {
+  LSH_REQUIRE(device >= 0 && device < kStaticDevices, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_address[device]) {
+    void* address = nullptr;
+    LSH_CHECK_HIP(hipGetSymbolAddress(&address, HIP_SYMBOL(scratch)),
+                  "hipGetSymbolAddress(scratch)");
+    g_scratch_address[device] = static_cast<lsh_tile*>(address);
+  }
+  *out = g_scratch_address[device];
+  return LSH_OK;
+}
+
+}  // namespace
+
+int lsh_lighting_tiles_static(lsh_session* session, int32_t dem_id, const lsh_tile* tiles,
+                              int32_t tile_count, const float* horizons, int32_t azimuth_count,
+                              int32_t azimuth_major, const double* sun, int32_t sun_count,
+                              const double* earth, int32_t earth_count,
+                              const double* psr_vectors, int32_t psr_count,
+                              double sun_half_angle_deg, double psr_diameter_deg, int32_t flags,
+                              uint8_t* sun_bytes, float* sun_fraction, float* sun_mean,
+                              float* earth_margin, float* earth_visible, uint8_t* psr,
+                              float* psr_margin) {
+  Session* s = reinterpret_cast<Session*>(session);
+  LSH_REQUIRE(s, "invalid session");
+  Dem* d = find_dem(s, dem_id);
+  LSH_REQUIRE(d, "unknown DEM id");
+  LSH_REQUIRE(tile_count <= kStaticTileCapacity,
+              "lsh_lighting_tiles_static takes at most 4096 tiles per call");
+  lsh::LightArgs a;
+  if (int r = light_args(tiles, tile_count, d->width, d->height, horizons, azimuth_count,
+                         azimuth_major, sun, sun_count, earth, earth_count, psr_vectors, psr_count,
+                         sun_half_angle_deg, psr_diameter_deg, flags, sun_bytes, sun_fraction,
+                         sun_mean, earth_margin, earth_visible, psr, psr_margin, &a))
+    return r;
+  a.elevation = d->level0;
+  std::memcpy(a.geo_transform, d->geo_transform, sizeof(a.geo_transform));
+  std::memcpy(a.projection, d->projection, sizeof(a.projection));
+  if (int r = select_device(s)) return r;
+  lsh_tile* table = nullptr;
+  if (int r = static_tile_table(s->device, &table)) return r;
+  std::lock_guard<std::mutex> lock(s->mutex);
+  // stream ordered: a previous call's kernel on this stream has read the table
+  // before this copy replaces it (calls on other streams of the device race)
+  LSH_CHECK_HIP(hipMemcpyAsync(table, tiles, sizeof(lsh_tile) * tile_count,
+                               hipMemcpyHostToDevice, s->stream),
+                "copy tile list");
+  a.tiles = table;
+  LSH_CHECK_HIP(lsh::launch_lighting(a, s->stream), "lighting kernel launch");
+  return LSH_OK;
+}
+
 int lsh_sun_loss(lsh_session* session, int32_t dem_id, const lsh_tile* tiles,
                  int32_t tile_count, const float* horizons, int32_t azimuth_count,
                  const double* sun, int32_t sun_count, double sun_half_angle_deg,
