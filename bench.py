@@ -12,12 +12,18 @@ Earth Moon-ME vectors at 6-hour steps over 31 days.
 A step is one batch of tiles (``--tiles-per-step``, default 24): FP64 segment
 generation, the fused two-pass march into HBM (degrees), and the streamed
 lighting pass over the batch. Tiles are taken in the order i * 97 mod 256, so
-any run of consecutive steps samples rows and columns of the whole site evenly;
-the default 4 timed steps cover 96 of the 256 tiles (``--steps 11 --tiles-per-step
-24`` or ``--steps 8 --tiles-per-step 32`` covers the site). DEMs and vectors are
-resident in HBM before the timed region.
+any run of consecutive steps samples rows and columns of the whole site evenly.
+DEMs and vectors are resident in HBM before the timed region.
 
-Prints one JSON line (see the fields below).
+Besides the headline the line carries: the march's roofline (FP64 VALU: the
+method's FLOPs per hierarchy test and per level-0 sample, counted by the
+kernel's counting build on timed tiles, over the HIP-event time), the lighting
+roofline of the 365-day pass, the batched persistent path on the 48-site
+workload in one launch against the per-tile launches, the end-to-end
+``generate_horizons`` path with compressed tile writes, the adjoint, and the
+CPU baselines.
+
+Prints one JSON line.
 """
 
 from __future__ import annotations
@@ -26,6 +32,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -35,10 +42,23 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "horizon tiles/s + lightmap px·times/s, 1 MI355X; batched vs plain at ≤0.01° err"
+FP64_VALU_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
-HORIZON_DEGREES_FLAG = 1
 AZIMUTHS = 1440
 PIXELS = 128 * 128
+# FP64 operations the method performs (the float64 oracle's hierarchy_maximum,
+# oracle/lsh_oracle.cpp): per hierarchy test, the position (two quartics, planar
+# distance, chord, footprint: 41) and the test (two tangents, cell exit, cull
+# slope: 54); per level-0 sample, the bilinear height, the slope and the
+# adaptive step (37). A square root or a division counts as one operation.
+MARCH_FLOP_PER_TEST = 95
+MARCH_FLOP_PER_SAMPLE = 37
+# lighting per pixel-time: Sun = frame transform, azimuth, elevation (27 FP64)
+# + the 16-slice float32 disk walk (100); Earth = frame + az/el + horizon
+# interpolation and margin (33). The PSR vectors are not counted (the mask
+# stops at the first lighting vector, so its work is data dependent).
+LIGHT_FLOP_SUN = 127
+LIGHT_FLOP_EARTH = 33
 
 
 def log(*args):
@@ -54,12 +74,13 @@ def parse_args():
     p.add_argument("--cpu-sample-pixels", type=int, default=256,
                    help="pixels of one tile the CPU baseline marches (0 disables it)")
     p.add_argument("--counter-tiles", type=int, default=4,
-                   help="tiles re-run with traversal counters for the roofline (0 skips)")
+                   help="timed tiles re-run with traversal counters for the roofline (0 skips)")
     p.add_argument("--no-probes", action="store_true",
                    help="skip the copy / gather peak probes")
-    p.add_argument("--batch-tiles", type=int, default=64,
+    p.add_argument("--batch-tiles", type=int, default=48,
                    help="tiles of the 48-site workload timed plain and batched (0 skips)")
-    p.add_argument("--batch-size", type=int, default=32, help="tiles per batched launch")
+    p.add_argument("--pipeline-tiles", type=int, default=12,
+                   help="tiles of the end-to-end generate_horizons leg (0 skips)")
     p.add_argument("--no-year-lighting", dest="year_lighting", action="store_false",
                    help="skip the 365-day lighting pass over resident horizons")
     p.add_argument("--no-adjoint", dest="adjoint", action="store_false",
@@ -113,9 +134,8 @@ def main() -> int:
             e0 = Event(session).record()
             segments = session.segments_device(dems, tile_column=c, tile_row=r, out=segments)
             e1 = Event(session).record()
-            view = _slot_view(horizons, slot)
             session.march_device(dems, segments, tile_column=c, tile_row=r,
-                                 flags=HORIZON_DEGREES, output=view)
+                                 flags=HORIZON_DEGREES, output=horizons.slot(slot))
             e2 = Event(session).record()
             events.append((e0, e1, e2))
         l0 = Event(session).record()
@@ -147,13 +167,11 @@ def main() -> int:
     march_ms = stats["march_ms"] / max(stats["march_launches"], 1)
     lighting_ms = stats["lighting_ms"] / max(stats["lighting_launches"], 1)
     light_px_times = per_step * PIXELS * (len(sun) + len(earth) + len(psr_vectors))
-
-    # the resident horizons are those of the last step's tiles
     last = args.warmup + args.steps - 1
     resident = [tiles[(last * per_step + i) % len(tiles)] for i in range(per_step)]
 
-    # traversal counters on the timed tiles -> algorithmic bytes per tile
-    S = T = bytes_alg = achieved = None
+    # traversal counters on timed tiles -> the method's FLOPs and gathered bytes per tile
+    counted = None
     if args.counter_tiles > 0:
         counters = DeviceBuffer.empty(session, (4,), np.int64)
         scratch = DeviceBuffer.empty(session, (PIXELS, AZIMUTHS), np.float32)
@@ -167,26 +185,31 @@ def main() -> int:
                                  output=scratch, counters=counters)
             v = counters.to_host()
             samples += int(v[0])
-            tests += int(v[1])
+            tests += int(v[2])  # every hierarchy test (culls, nodata skips, descents, samples)
         scratch.free()
         counters.free()
-        rays = PIXELS * AZIMUTHS
-        S, T = samples / len(pick), tests / len(pick)
-        passes = len(dems)
-        bytes_alg = 16.0 * (S + T) + 256.0 * rays * passes + 4.0 * rays
-        achieved = bytes_alg / (march_ms / 1000.0) / 1e9
-    traffic = _traffic_from_profile()
+        counted = {"tiles": len(pick), "samples": samples / len(pick), "tests": tests / len(pick)}
+    roofline = _march_roofline(counted, march_ms, len(dems))
     codec = _codec_leg(horizons)
+    cpu_light_tile = horizons.slot(0).to_host()
     long_lighting = reducers = None
     if args.year_lighting:
         long_lighting = _year_lighting(session, lighting, primary, horizons, resident)
         reducers = _durations_leg(session, lighting, primary, horizons, resident)
-    peaks = None
+    horizons.free()
+    if segments is not None:
+        segments.free()
+    if outputs is not None:
+        outputs.free()
     if not args.no_probes:
         from lunarscout_amd.profiling import probe_peaks
 
-        horizons.free()
         peaks = probe_peaks(session)
+        roofline["measured_peaks"] = peaks
+        if roofline.get("gathered_bytes_per_launch"):
+            roofline["gather_frac_of_l2_peak"] = round(
+                roofline["gathered_bytes_per_launch"] / (march_ms / 1000.0) / 1e9
+                / peaks["gather_l2_gbs"], 4)
 
     result = {
         "metric": METRIC,
@@ -210,8 +233,7 @@ def main() -> int:
         "horizon": {
             "march_ms_per_tile": round(march_ms, 3),
             "segments_ms_per_tile": round(stats["segments_ms"] / max(stats["march_launches"], 1), 3),
-            "level0_samples_per_tile": S,
-            "bound_tests_per_tile": T,
+            "counted": counted,
         },
         "tile_codec": codec,
         "lighting": {
@@ -222,35 +244,60 @@ def main() -> int:
             "year": long_lighting,
             "year_reducers": reducers,
         },
-        "roofline": {
-            "kernel": "march_kernel",
-            "bound": "hbm",
-            "achieved": round(achieved, 1) if achieved is not None else None,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved is not None else None,
-            "traffic": traffic,
-            "bytes_alg_per_launch": bytes_alg,
-            "measured_peaks": peaks,
-            "frac_of_l2_gather": (round(achieved / peaks["gather_l2_gbs"], 4)
-                                  if peaks and achieved is not None else None),
-        },
+        "roofline": roofline,
     }
     if args.batch_tiles > 0:
-        result["batched"] = _batched_vs_plain(session, dems[1], args.batch_tiles, args.batch_size)
+        result["batched"] = _batched_vs_plain(session, dems[1], args.batch_tiles)
+    if args.pipeline_tiles > 0:
+        result["generate_horizons"] = _pipeline_leg(session, dems, tiles[:args.pipeline_tiles])
     if args.adjoint:
         result["adjoint"] = _adjoint_leg(session)
     if args.cpu_sample_pixels > 0:
-        result["cpu_baseline"] = _cpu_baseline(dems, sun, args.cpu_sample_pixels)
+        result["cpu_baseline"] = _cpu_baseline(dems, args.cpu_sample_pixels, cpu_light_tile,
+                                               resident[0])
     print(json.dumps(result), flush=True)
     return 0
+
+
+def _profile_summary(name: str):
+    path = ROOT / "profiles" / name
+    if not path.exists():
+        return None
+    try:
+        return json.loads(path.read_text())
+    except (OSError, ValueError):
+        return None
+
+
+def _march_roofline(counted, march_ms: float, passes: int) -> dict:
+    """FP64 VALU roofline of the march from the counted tests and samples."""
+    pmc = _profile_summary("march_full_site_pmc.json") or {}
+    out = {"kernel": "march_kernel", "bound": "valu", "unit": "TFLOP/s",
+           "peak": FP64_VALU_PEAK_TFLOPS, "achieved": None, "frac": None,
+           "traffic": pmc.get("hbm_bytes_per_launch"),
+           "flop_model": {"per_hierarchy_test": MARCH_FLOP_PER_TEST,
+                          "per_level0_sample": MARCH_FLOP_PER_SAMPLE},
+           "valu_busy_profile": pmc.get("valu_busy"),
+           "fp64_tflops_profile": pmc.get("fp64_tflops")}
+    if counted:
+        flops = MARCH_FLOP_PER_TEST * counted["tests"] + MARCH_FLOP_PER_SAMPLE * counted["samples"]
+        achieved = flops / (march_ms / 1000.0) / 1e12
+        rays = PIXELS * AZIMUTHS
+        # bytes the kernel gathers: one stored 4-byte bound per hierarchy test,
+        # the four 4-byte level-0 taps per sample, the segment corners and the output
+        gathered = 4.0 * counted["tests"] + 16.0 * counted["samples"] + 256.0 * rays * passes \
+            + 4.0 * rays
+        out.update({"achieved": round(achieved, 3), "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4),
+                    "flop_per_launch": flops, "gathered_bytes_per_launch": gathered,
+                    "hbm_frac_of_gathered": round(gathered / (march_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)})
+    return out
 
 
 def _codec_leg(horizons):
     """.cbin encode / decode of one marched full-site tile on the host threads."""
     from lunarscout_amd import tiles
 
-    degrees = _slot_view(horizons, 0).to_host()
+    degrees = horizons.slot(0).to_host()
     payload = tiles.encode_cbin(degrees)
     t0 = time.perf_counter()
     for _ in range(3):
@@ -288,11 +335,22 @@ def _year_lighting(session, lighting, primary, horizons, batch):
     outputs.free()
     for b in bufs:
         b.free()
-    px_times = len(batch) * PIXELS * (len(sun) + len(earth) + len(psr_vectors))
+    pixels = len(batch) * PIXELS
+    px_times = pixels * (len(sun) + len(earth) + len(psr_vectors))
+    flops = pixels * (LIGHT_FLOP_SUN * len(sun) + LIGHT_FLOP_EARTH * len(earth))
+    achieved = flops / (ms / 1000.0) / 1e12
     return {"workload": f"{len(batch)} resident full-site tiles x (4380 Sun + 4380 Earth + "
                         f"{len(psr_vectors)} PSR vectors), 2-hour steps over 365 days",
             "value": round(px_times / (ms / 1000.0), 1), "unit": "px·times/s",
-            "kernel_ms": round(ms, 3)}
+            "kernel_ms": round(ms, 3),
+            "roofline": {"kernel": "lighting_kernel", "bound": "valu", "unit": "TFLOP/s",
+                         "achieved": round(achieved, 3), "peak": FP64_VALU_PEAK_TFLOPS,
+                         "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4),
+                         "flop_model": {"per_sun_px_time": LIGHT_FLOP_SUN,
+                                        "per_earth_px_time": LIGHT_FLOP_EARTH,
+                                        "psr": "not counted"},
+                         "valu_busy_profile": (_profile_summary("lighting_year_pmc.json") or {})
+                         .get("valu_busy")}}
 
 
 def _durations_leg(session, lighting, primary, horizons, batch):
@@ -350,7 +408,8 @@ def _adjoint_leg(session):
     dems = synthetic.site_dems(synthetic.ADJOINT_SITE)
     tiles = [(x, y) for y in range(0, 512, 128) for x in range(0, 512, 128)]
     sun = synthetic.sun_vectors(dems[0], 2190, 4.0, seed=10)
-    problem = SunLossProblem(dems, tiles, sun, azimuth_count=720, session=session)
+    problem = SunLossProblem(dems, tiles, sun, azimuth_count=720, smoothing_deg=0.08,
+                             session=session)
     problem.forward()
     problem.backward()
     t0 = time.perf_counter()
@@ -362,7 +421,8 @@ def _adjoint_leg(session):
     sec = dict(problem.seconds)
     problem.close()
     return {
-        "workload": "512^2 @10 m + 1024^2 @60 m, 16 tiles x 720 az, 2190 Sun vectors (4-hourly, 365 d)",
+        "workload": "512^2 @10 m + 1024^2 @60 m, 16 tiles x 720 az, 2190 Sun vectors (4-hourly, "
+                    "365 d), slice clamps rounded over 0.08 deg",
         "loss": loss,
         "forward_s": round(forward_s, 4), "backward_s": round(backward_s, 4),
         "backward_over_forward": round(backward_s / forward_s, 4),
@@ -371,125 +431,156 @@ def _adjoint_leg(session):
     }
 
 
-def _batched_vs_plain(session, regional, tile_count, batch_size):
-    """48 landing sites sharing the regional DEM: plain per-tile launches vs the
-    persistent batched launch (hipGraph, autotuned shape), same tiles, same run."""
+def _batched_vs_plain(session, regional, tile_count):
+    """48 landing sites sharing the regional DEM: per-tile launches against one
+    persistent launch (hipGraph, autotuned shape) over the same tiles."""
+    import oracle
     from lunarscout_amd import synthetic
-    from lunarscout_amd._lib import DeviceBuffer, Event
+    from lunarscout_amd._lib import HORIZON_DEGREES, DeviceBuffer, Event
     from lunarscout_amd.batch import HorizonBatch, Site, autotune
 
     t0 = time.time()
     sites = synthetic.landing_sites()
     site_dems = [synthetic.landing_site_dem(s) for s in sites]
-    work = []  # the first n tiles (row-major) of every patch
+    work = []  # every tile of every patch
     for i, s in enumerate(sites):
         for k in range(s.tile_count):
             work.append((i, (k % s.tiles_x) * 128, (k // s.tiles_x) * 128))
     pick = np.linspace(0, len(work) - 1, min(tile_count, len(work))).astype(int)
     sample = [work[i] for i in pick]
+    n = len(sample)
     batch_sites = [Site((d, regional)) for d in site_dems]
-    log(f"48-site workload: {len(work)} tiles, timing {len(sample)} ({time.time() - t0:.1f}s setup)")
-    batch = HorizonBatch(session, batch_sites, max_tiles=batch_size)
-    out = DeviceBuffer.empty(session, (batch_size, PIXELS, AZIMUTHS), np.float32)
-    tune = autotune(batch, sample[: min(8, len(sample))], out, key="landing-sites-48",
-                    use_cache=False)
+    for d in site_dems:
+        session.dem_id(d)
+    log(f"48-site workload: {len(work)} tiles, timing {n} ({time.time() - t0:.1f}s setup)")
+    batch = HorizonBatch(session, batch_sites, max_tiles=n)
+    out = DeviceBuffer.empty(session, (n, PIXELS, AZIMUTHS), np.float32)
+    tune = autotune(batch, sample[: min(8, n)], out, key="landing-sites-48", use_cache=False)
     log(f"autotune: shape {tune['config']} workgroups {tune['workgroups']}")
-    # plain: one segments + one march launch per tile
+    # plain: one segments + one march launch per tile, each into its own slot
+    plain = DeviceBuffer.empty(session, (n, PIXELS, AZIMUTHS), np.float32)
     segments = None
-    plain_out = DeviceBuffer.empty(session, (PIXELS, AZIMUTHS), np.float32)
+    kernel_ms = 0.0
     session.synchronize()
-    e0 = Event(session).record()
-    for site, c, r in sample:
+    w0 = Event(session).record()
+    marks = []
+    for slot, (site, c, r) in enumerate(sample):
         dl = [site_dems[site], regional]
+        e0 = Event(session).record()
         segments = session.segments_device(dl, tile_column=c, tile_row=r, out=segments)
-        session.march_device(dl, segments, tile_column=c, tile_row=r, flags=HORIZON_DEGREES_FLAG,
-                             output=plain_out)
-    e1 = Event(session).record()
+        session.march_device(dl, segments, tile_column=c, tile_row=r, flags=HORIZON_DEGREES,
+                             output=plain.slot(slot))
+        e1 = Event(session).record()
+        marks.append((e0, e1))
+    w1 = Event(session).record()
     session.synchronize()
-    plain_ms = e0.elapsed_ms(e1)
-    log(f"plain: {plain_ms / len(sample):.1f} ms/tile")
-    # batched: persistent launches of batch_size tiles, graph replay
+    plain_ms = w0.elapsed_ms(w1)
+    kernel_ms = sum(a.elapsed_ms(b) for a, b in marks)
+    log(f"plain: {plain_ms / n:.1f} ms/tile, GPU busy {kernel_ms / plain_ms:.4f}")
+    # batched: all sampled tiles in one persistent launch (graph replay after the capture)
+    batch.run(sample, out, config=tune["config"], workgroups=tune["workgroups"], degrees=True,
+              use_graph=True)
+    session.synchronize()
     e0 = Event(session).record()
-    for start in range(0, len(sample), batch_size):
-        chunk = sample[start:start + batch_size]
-        batch.run(chunk, out, config=tune["config"], workgroups=tune["workgroups"], degrees=True,
-                  use_graph=True)
+    batch.run(sample, out, config=tune["config"], workgroups=tune["workgroups"], degrees=True,
+              use_graph=True)
     e1 = Event(session).record()
     session.synchronize()
     batched_ms = e0.elapsed_ms(e1)
-    log(f"batched: {batched_ms / len(sample):.1f} ms/tile")
-    # the last chunk's first tile, plain, must equal the batched result bit for bit
-    site, c, r = sample[(len(sample) - 1) // batch_size * batch_size]
+    log(f"batched: {batched_ms / n:.1f} ms/tile")
+    identical = 0
+    for slot in range(n):
+        a, b = plain.slot(slot).to_host(), out.slot(slot).to_host()
+        identical += int(np.all((a == b) | (np.isneginf(a) & np.isneginf(b))))
+    # the batched result against the float64 oracle on 64 pixels of the first sampled tile
+    site, c, r = sample[0]
     dl = [site_dems[site], regional]
-    segments = session.segments_device(dl, tile_column=c, tile_row=r, out=segments)
-    session.march_device(dl, segments, tile_column=c, tile_row=r, flags=HORIZON_DEGREES_FLAG,
-                         output=plain_out)
-    a, b = plain_out.to_host(), _slot_view(out, 0).to_host()
-    identical = bool(np.all((a == b) | (np.isneginf(a) & np.isneginf(b))))
+    px = np.linspace(0, PIXELS - 1, 64).astype(np.int32)
+    segs = oracle.segments(dl, tile_column=c, tile_row=r, tile_width=128, azimuth_count=AZIMUTHS)
+    ref, _ = oracle.horizon_tile([oracle.OraclePyramid(d) for d in dl], segs, tile_column=c,
+                                 tile_row=r, pixels=px, float64=True)
+    got = out.slot(0).to_host()[px].astype(np.float64)
+    ref_deg = np.degrees(np.arctan(ref))
+    finite = np.isfinite(ref_deg)
+    err = float(np.max(np.abs(got - ref_deg)[finite])) if finite.any() else 0.0
     batch.close()
     out.free()
-    plain_out.free()
+    plain.free()
+    if segments is not None:
+        segments.free()
+    busy = kernel_ms / plain_ms
     return {
         "workload": "48 landing-site patches (tile counts log-uniform 1..64, 5-20 m/px, seeds "
                     "100-147, centres within +-60 km) sharing the 4096^2 @60 m regional DEM; "
-                    f"{len(work)} tiles, {len(sample)} timed (evenly spaced over the sites)",
-        "plain_tiles_per_s": round(len(sample) / (plain_ms / 1000.0), 4),
-        "batched_tiles_per_s": round(len(sample) / (batched_ms / 1000.0), 4),
+                    f"{len(work)} tiles, {n} timed (evenly spaced over the sites)",
+        "plain_tiles_per_s": round(n / (plain_ms / 1000.0), 4),
+        "batched_tiles_per_s": round(n / (batched_ms / 1000.0), 4),
         "speedup": round(plain_ms / batched_ms, 4),
-        "batch_size": batch_size,
+        "batched_launches": 1,
+        "plain_gpu_busy_fraction": round(busy, 5),
+        "plain_launch_gap_ms_per_tile": round((plain_ms - kernel_ms) / n, 4),
         "autotuned": {"shape": tune["config"], "workgroups": tune["workgroups"],
                       "trials": [(t["shape"], t["workgroups"], round(t["ms"], 1))
                                  for t in tune["trials"]]},
-        "bit_identical_to_plain": identical,
-        "max_err_deg_vs_plain": 0.0 if identical else None,
+        "bit_identical_tiles": f"{identical}/{n}",
+        "max_err_deg_vs_oracle": {"value": err, "sample": "64 px x 1440 az of the first timed tile"},
     }
 
 
-def _slot_view(buffer, slot):
-    """A DeviceBuffer-like view of one tile slot of a [slots, pixels, N] buffer."""
-    from lunarscout_amd._lib import DeviceBuffer
+def _pipeline_leg(session, dems, tiles):
+    """generate_horizons' path end to end on a subset of the full site: the
+    bounded pipeline (GPU segments + march, degrees copied back, .cbin encode
+    and the staged, fsynced tile publish in the writer thread) into a fresh
+    directory; tiles/s from the wall clock."""
+    from lunarscout_amd.pipeline import PatchDescriptor, run_bounded_pipeline
+    from lunarscout_amd.tiles import HorizonTileStore
 
-    view = DeviceBuffer.__new__(DeviceBuffer)
-    view.session = buffer.session
-    view.dtype = buffer.dtype
-    view.shape = buffer.shape[1:]
-    view.nbytes = int(np.prod(view.shape)) * view.dtype.itemsize
-    view.ptr = buffer.ptr + slot * view.nbytes
-    view.free = lambda: None  # the parent owns the memory
-    return view
+    patches = [PatchDescriptor(i, c, r, c // 128, r // 128, 128, 128) for i, (c, r) in enumerate(tiles)]
+
+    def factory(_worker):
+        def process(patch, _prepared):
+            return session.horizon_tile(dems, tile_column=patch.tile_x, tile_row=patch.tile_y,
+                                        degrees=True)
+        return process
+
+    with tempfile.TemporaryDirectory(prefix="lsh-bench-") as root:
+        started = time.perf_counter()
+        result = run_bounded_pipeline(patches, store=HorizonTileStore(root),
+                                      prepare_patch=lambda _p: None, processor_factory=factory,
+                                      compress=True, skip_existing=False,
+                                      prepared_queue_capacity=1, writer_queue_capacity=1)
+        wall = time.perf_counter() - started
+        size = sum(p.stat().st_size for p in Path(root).rglob("*.cbin"))
+    return {"workload": f"{len(tiles)} full-site tiles through generate_horizons' bounded "
+                        "pipeline, compressed (.cbin) staged writes",
+            "tiles_per_s": round(len(tiles) / wall, 4), "wall_s": round(wall, 3),
+            "compute_s": round(result.compute_seconds, 3), "write_s": round(result.write_seconds, 3),
+            "bytes_written": int(size)}
 
 
-def _traffic_from_profile():
-    """HBM bytes per march launch from the committed rocprofv3 PMC summary, if present."""
-    path = ROOT / "profiles" / "march_full_site_pmc.json"
-    if not path.exists():
-        return None
-    try:
-        return json.loads(path.read_text()).get("hbm_bytes_per_launch")
-    except (OSError, ValueError):
-        return None
-
-
-def _cpu_baseline(dems, sun, sample_pixels):
-    """The C++/OpenMP oracle (float32 reference arithmetic) on a bounded sample."""
+def _cpu_baseline(dems, sample_pixels, horizon_tile, tile):
+    """The C++/OpenMP oracle (float32 reference arithmetic) on bounded samples."""
     import oracle
 
     cores = oracle.cpu_threads()
     pyramids = [oracle.OraclePyramid(d) for d in dems]
-    tile = (1024, 1024)
+    march_tile = (1024, 1024)
     t0 = time.perf_counter()
-    segs = oracle.segments(dems, tile_column=tile[0], tile_row=tile[1], tile_width=128,
+    segs = oracle.segments(dems, tile_column=march_tile[0], tile_row=march_tile[1], tile_width=128,
                            azimuth_count=AZIMUTHS, threads=cores)
     t_seg = time.perf_counter() - t0
     pixels = np.linspace(0, PIXELS - 1, sample_pixels).astype(np.int32)
     t0 = time.perf_counter()
-    oracle.horizon_tile(pyramids, segs, tile_column=tile[0], tile_row=tile[1], pixels=pixels,
-                        mode=oracle.FLOAT32 | oracle.REFERENCE_SLOPE, threads=cores)
+    oracle.horizon_tile(pyramids, segs, tile_column=march_tile[0], tile_row=march_tile[1],
+                        pixels=pixels, mode=oracle.FLOAT32 | oracle.REFERENCE_SLOPE, threads=cores)
     t_march = time.perf_counter() - t0
     per_tile = t_seg + t_march * PIXELS / sample_pixels
-    horizons = np.zeros((128, 128, AZIMUTHS), np.float32)
+    from lunarscout_amd import synthetic
+
+    sun = synthetic.sun_vectors(dems[0], 4380, 2.0, seed=5)
+    rows = np.maximum(horizon_tile, np.float32(-50.0)).reshape(128, 128, AZIMUTHS)
     t0 = time.perf_counter()
-    oracle.lighting_patch(dems[0], horizons, sun, tile_y=tile[1], tile_x=tile[0],
+    oracle.lighting_patch(dems[0], rows, sun, tile_y=tile[1], tile_x=tile[0],
                           float32_slices=True, threads=cores)
     t_light = time.perf_counter() - t0
     return {
@@ -497,9 +588,10 @@ def _cpu_baseline(dems, sun, sample_pixels):
         "unit": "tiles/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"tile {tile}: full FP64 segment generation ({t_seg:.2f}s) + float32 march of "
-                  f"{sample_pixels} of 16384 px x 1440 az x 2 passes ({t_march:.2f}s), scaled "
-                  f"to one tile; lighting: one tile x {len(sun)} Sun times ({t_light:.2f}s)",
+        "sample": f"tile {march_tile}: full FP64 segment generation ({t_seg:.2f}s) + float32 march "
+                  f"of {sample_pixels} of 16384 px x 1440 az x 2 passes ({t_march:.2f}s), scaled "
+                  f"to one tile; lighting: marched full-site tile {tile} x 4380 Sun times "
+                  f"({t_light:.2f}s)",
         "lighting_px_times_per_s": round(PIXELS * len(sun) / t_light, 1),
     }
 

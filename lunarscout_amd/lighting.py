@@ -23,6 +23,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import DeviceBuffer, check
+from .errors import InputError
 from .geometry import DemGrid, projection_array
 
 PATCH_SIZE = 128
@@ -79,6 +80,41 @@ def _tile_array(tiles) -> np.ndarray:
     return arr  # (tile_column, tile_row) pairs == lsh_tile
 
 
+def _check_device_horizons(horizons, tile_count: int, azimuth_major: bool) -> int:
+    """The azimuth count of a device horizon buffer holding ``tile_count`` tiles.
+
+    The buffer must be float32 [tile][16384][N] (or [tile][N][16384]
+    azimuth-major) with at least ``tile_count`` tiles, so no kernel reads past it.
+    """
+    if not isinstance(horizons, DeviceBuffer):
+        raise InputError("horizons must be a device buffer", code="lighting_input_invalid")
+    if horizons.dtype != np.float32 or len(horizons.shape) < 2:
+        raise InputError("horizons must be float32 [tile][pixel][azimuth] on the device",
+                         code="lighting_input_invalid", details={"dtype": str(horizons.dtype)})
+    N = int(horizons.shape[-2] if azimuth_major else horizons.shape[-1])
+    pixels = int(horizons.shape[-1] if azimuth_major else horizons.shape[-2])
+    if pixels != PATCH_PIXELS or N < 2:
+        raise InputError("horizon tiles must hold 16384 pixels and at least 2 azimuths",
+                         code="lighting_input_invalid", details={"shape": list(horizons.shape)})
+    if horizons.nbytes < tile_count * PATCH_PIXELS * N * 4:
+        raise InputError("the horizon buffer holds fewer tiles than requested",
+                         code="lighting_input_invalid",
+                         details={"tiles": tile_count, "shape": list(horizons.shape)})
+    return N
+
+
+def _check_device_vectors(buf, name: str, minimum: int = 0) -> int:
+    """The vector count of a device [T][3] float64 buffer (at least ``minimum``)."""
+    if buf.dtype != np.float64 or len(buf.shape) != 2 or buf.shape[1] != 3:
+        raise InputError(f"{name} vectors must be float64 [T][3] on the device",
+                         code="lighting_input_invalid", details={"shape": list(buf.shape)})
+    if buf.nbytes < buf.shape[0] * 24 or buf.shape[0] < minimum:
+        raise InputError(f"{name} vectors hold fewer entries than required",
+                         code="lighting_input_invalid",
+                         details={"have": int(buf.shape[0]), "need": int(minimum)})
+    return int(buf.shape[0])
+
+
 class HipLightingSession:
     """Lightmap / margin / PSR on the GPU with fixed-shape reusable buffers."""
 
@@ -108,7 +144,7 @@ class HipLightingSession:
         dem_id = s.dem_id(dem)
         tile_arr = _tile_array(tiles)
         n = len(tile_arr)
-        N = horizons.shape[-1] if not azimuth_major else horizons.shape[-2]
+        N = _check_device_horizons(horizons, n, azimuth_major)
         bufs = {}
         counts = {}
         for name, vec in (("sun", sun), ("earth", earth), ("psr", psr_vectors)):
@@ -117,7 +153,7 @@ class HipLightingSession:
                 counts[name] = 0
             elif isinstance(vec, DeviceBuffer):
                 bufs[name] = vec
-                counts[name] = vec.shape[0]
+                counts[name] = _check_device_vectors(vec, name)
             else:
                 v = validate_vectors(vec)
                 bufs[name] = DeviceBuffer.from_host(s, v)
@@ -170,18 +206,29 @@ class HipLightingSession:
         dem_id = s.dem_id(dem)
         tile_arr = _tile_array(tiles)
         n = len(tile_arr)
-        N = horizons.shape[-2] if azimuth_major else horizons.shape[-1]
+        N = _check_device_horizons(horizons, n, azimuth_major)
         spec = plan.spec()
         owned = []
 
-        def device(vec):
-            if vec is None or isinstance(vec, DeviceBuffer):
+        def device(vec, name):
+            if vec is None:
                 return vec
-            buf = DeviceBuffer.from_host(s, validate_vectors(vec))
+            if isinstance(vec, DeviceBuffer):
+                _check_device_vectors(vec, name, plan.time_count)
+                return vec
+            host = validate_vectors(vec)
+            if len(host) < plan.time_count:
+                raise InputError(f"{name} vectors hold fewer entries than the plan's times",
+                                 code="lighting_input_invalid",
+                                 details={"have": len(host), "need": plan.time_count})
+            buf = DeviceBuffer.from_host(s, host)
             owned.append(buf)
             return buf
 
-        sun_buf, earth_buf = device(sun), device(earth if plan.needs_earth else None)
+        if plan.needs_earth and earth is None:
+            raise InputError("this duration plan needs Earth vectors", code="lighting_input_invalid")
+        sun_buf = device(sun, "sun")
+        earth_buf = device(earth if plan.needs_earth else None, "earth")
         need = ctypes.c_int64()
         check(s.lib.lsh_duration_workspace_bytes(ctypes.byref(spec), n, ctypes.byref(need)),
               "lsh_duration_workspace_bytes")

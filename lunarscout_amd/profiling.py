@@ -13,6 +13,8 @@ Command line:
   full-site march on K tiles (segments + one march launch each, degrees) and,
   with T > 0, one lighting launch over them with T Sun + T Earth vectors
   (2-hour steps) plus their PSR reduction -- a short program for counter runs;
+* ``python -m lunarscout_amd.profiling counters ...`` turns the ``--pmc`` passes
+  of one kernel into VALU busy, FP64 rate, L2 hit rate and scaled HBM bytes;
 * ``python -m lunarscout_amd.profiling summarize ...`` turns rocprofv3 CSV
   output (kernel trace stats and the FETCH_SIZE / WRITE_SIZE passes) into the
   short text and JSON summaries kept with the sources.
@@ -162,6 +164,99 @@ def kernel_stats(directory: Path) -> list[dict]:
     return list(_rows(_find(directory, "kernel_stats.csv")))
 
 
+def _kernel_counters(directory: Path, kernel: str) -> tuple[dict, int, float]:
+    """Mean per-dispatch counter values of kernels whose name contains ``kernel``.
+
+    Returns ({counter: mean}, dispatch count, mean dispatch duration in ms).
+    """
+    per: dict[str, dict[str, float]] = {}
+    spans: dict[str, float] = {}
+    for row in _rows(_find(directory, "counter_collection.csv")):
+        if kernel not in row["Kernel_Name"]:
+            continue
+        key = row.get("Dispatch_Id") or row.get("Correlation_Id")
+        per.setdefault(key, {})
+        per[key][row["Counter_Name"]] = per[key].get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+        spans[key] = (int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) / 1e6
+    if not per:
+        raise ValueError(f"no dispatch of {kernel} under {directory}")
+    names = set().union(*per.values())
+    means = {n: float(np.mean([d[n] for d in per.values() if n in d])) for n in names}
+    return means, len(per), float(np.mean(list(spans.values())))
+
+
+def counters(args) -> int:
+    """Counter summary of one kernel: VALU busy, FP64 rate, L2 hits, HBM bytes."""
+    merged: dict[str, float] = {}
+    dispatches = []
+    durations = []
+    for directory in args.pmc:
+        means, n, ms = _kernel_counters(Path(directory), args.kernel)
+        merged.update(means)
+        dispatches.append(n)
+        durations.append(ms)
+    duration_ms = float(np.mean(durations))
+    cus, xccs = args.cu_count, args.xcc_count
+    out: dict = {"kernel": args.kernel, "workload": args.workload,
+                 "dispatches_per_pass": dispatches, "duration_ms": round(duration_ms, 3),
+                 "pmc_commands": args.pmc_command}
+    gui = merged.get("GRBM_GUI_ACTIVE")
+    if gui and "SQ_ACTIVE_INST_VALU" in merged:
+        # VALUBusy: VALU quad-cycles summed over the SIMDs of every CU, over one
+        # XCC's busy cycles (GRBM_GUI_ACTIVE is summed over the XCCs)
+        out["valu_busy"] = round(merged["SQ_ACTIVE_INST_VALU"] / cus / (gui / xccs), 4)
+    if "SQ_THREAD_CYCLES_VALU" in merged and "SQ_ACTIVE_INST_VALU" in merged:
+        out["valu_lane_utilization"] = round(
+            merged["SQ_THREAD_CYCLES_VALU"] / (merged["SQ_ACTIVE_INST_VALU"] * 64.0), 4)
+    if "SQ_INSTS_VALU" in merged:
+        out["valu_instructions_per_dispatch"] = merged["SQ_INSTS_VALU"]
+    fp64 = [merged.get(k) for k in ("SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_MUL_F64",
+                                    "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_TRANS_F64")]
+    if all(v is not None for v in fp64):
+        flops = (2.0 * fp64[0] + fp64[1] + fp64[2] + fp64[3]) * 64.0
+        out["fp64_flop_per_dispatch"] = flops
+        out["fp64_tflops"] = round(flops / (duration_ms / 1000.0) / 1e12, 3)
+        out["fp64_frac_of_peak"] = round(out["fp64_tflops"] / 78.6, 4)
+        if "SQ_INSTS_VALU" in merged:
+            out["fp64_share_of_valu_instructions"] = round(sum(fp64) / merged["SQ_INSTS_VALU"], 4)
+    fp32 = [merged.get(k) for k in ("SQ_INSTS_VALU_FMA_F32", "SQ_INSTS_VALU_MUL_F32",
+                                    "SQ_INSTS_VALU_ADD_F32", "SQ_INSTS_VALU_TRANS_F32")]
+    if all(v is not None for v in fp32):
+        out["fp32_flop_per_dispatch"] = (2.0 * fp32[0] + fp32[1] + fp32[2] + fp32[3]) * 64.0
+    if "TCC_HIT_sum" in merged and "TCC_MISS_sum" in merged:
+        total = merged["TCC_HIT_sum"] + merged["TCC_MISS_sum"]
+        out["l2_hit_rate"] = round(merged["TCC_HIT_sum"] / total, 4) if total else None
+        out["l2_requests_per_dispatch"] = total
+    if args.fetch and args.write:
+        copy_bytes = args.copy_mib * 2**20
+        copy_fetch_kib, _ = counter_per_dispatch(Path(args.copy_fetch), COPY_KERNEL, "FETCH_SIZE")
+        copy_write_kib, _ = counter_per_dispatch(Path(args.copy_write), COPY_KERNEL, "WRITE_SIZE")
+        fetch_scale = copy_bytes / (copy_fetch_kib * 1024.0)
+        write_scale = copy_bytes / (copy_write_kib * 1024.0)
+        fetch_kib, _ = counter_per_dispatch(Path(args.fetch), args.kernel, "FETCH_SIZE")
+        write_kib, _ = counter_per_dispatch(Path(args.write), args.kernel, "WRITE_SIZE")
+        out.update({"fetch_scale": round(fetch_scale, 4), "write_scale": round(write_scale, 4),
+                    "hbm_fetch_bytes_per_launch": fetch_kib * 1024.0 * fetch_scale,
+                    "hbm_write_bytes_per_launch": write_kib * 1024.0 * write_scale})
+        out["hbm_bytes_per_launch"] = out["hbm_fetch_bytes_per_launch"] + out["hbm_write_bytes_per_launch"]
+        out["hbm_gbs"] = round(out["hbm_bytes_per_launch"] / (duration_ms / 1000.0) / 1e9, 1)
+    lines = [f"# {args.title}", "", f"Workload: {args.workload}", "", "Commands:", ""]
+    lines += [f"    {c}" for c in args.pmc_command]
+    lines += ["", f"Mean per dispatch of `{args.kernel}` (duration {duration_ms:.3f} ms):", ""]
+    lines += ["| figure | value |", "|---|---|"]
+    lines += [f"| {k} | {v} |" for k, v in out.items()
+              if k not in ("kernel", "workload", "pmc_commands", "dispatches_per_pass")]
+    lines += ["", "Raw counter means per dispatch:", "", "| counter | value |", "|---|---|"]
+    lines += [f"| {k} | {merged[k]:.6g} |" for k in sorted(merged)]
+    if args.note:
+        lines += ["", args.note]
+    Path(args.out_text).write_text("\n".join(lines) + "\n")
+    if args.out_json:
+        Path(args.out_json).write_text(json.dumps(out, indent=1) + "\n")
+    print("\n".join(lines))
+    return 0
+
+
 def summarize(args) -> int:
     stats = kernel_stats(Path(args.trace))
     lines = [f"# {args.title}", "", "Command:", "", f"    {args.trace_command}", "",
@@ -236,6 +331,22 @@ def main(argv=None) -> int:
     w.add_argument("--times", type=int, default=4380)
     w.add_argument("--azimuth-major", action="store_true",
                    help="march into and light from [azimuth][pixel] horizons")
+    k = sub.add_parser("counters", help="summarize --pmc passes of one kernel")
+    k.add_argument("--kernel", required=True, help="substring of the kernel name")
+    k.add_argument("--pmc", action="append", required=True, help="directory of one --pmc pass")
+    k.add_argument("--pmc-command", action="append", default=[])
+    k.add_argument("--fetch", help="directory of the kernel's FETCH_SIZE pass")
+    k.add_argument("--write", help="directory of the kernel's WRITE_SIZE pass")
+    k.add_argument("--copy-fetch", help="directory of the copy probe's FETCH_SIZE pass")
+    k.add_argument("--copy-write", help="directory of the copy probe's WRITE_SIZE pass")
+    k.add_argument("--copy-mib", type=int, default=1024)
+    k.add_argument("--cu-count", type=int, default=256)
+    k.add_argument("--xcc-count", type=int, default=8)
+    k.add_argument("--workload", default="")
+    k.add_argument("--title", default="counters")
+    k.add_argument("--note", default="")
+    k.add_argument("--out-text", required=True)
+    k.add_argument("--out-json")
     s = sub.add_parser("summarize", help="summarize rocprofv3 CSV output")
     s.add_argument("--trace", required=True, help="directory of the --kernel-trace --stats run")
     s.add_argument("--trace-command", required=True)
@@ -255,6 +366,8 @@ def main(argv=None) -> int:
         return _copy(args.mib)
     if args.command == "workload":
         return _workload(args)
+    if args.command == "counters":
+        return counters(args)
     return summarize(args)
 
 

@@ -141,3 +141,34 @@ def test_disk_walk_equals_the_reference_walk_bit_for_bit_on_steep_rows():
                      for a, e in zip(az, el)], dtype=np.float32)
     assert np.count_nonzero(want == 1.0) > 10 and np.count_nonzero(want == 0.0) > 10
     np.testing.assert_array_equal(got, want)
+
+
+def _fake_device_buffer(shape, dtype):
+    """A DeviceBuffer descriptor with no device memory (the checks read only shape/dtype)."""
+    from lunarscout_amd._lib import DeviceBuffer
+
+    buf = DeviceBuffer.__new__(DeviceBuffer)
+    buf.session, buf.ptr = None, 0
+    buf.dtype, buf.shape = np.dtype(dtype), tuple(shape)
+    buf.nbytes = int(np.prod(shape)) * buf.dtype.itemsize
+    buf.free = lambda: None
+    return buf
+
+
+def test_device_inputs_are_checked_before_any_kernel_reads_them():
+    from lunarscout_amd.errors import InputError
+    from lunarscout_amd.lighting import _check_device_horizons, _check_device_vectors
+
+    assert _check_device_horizons(_fake_device_buffer((3, 16384, 1440), np.float32), 3, False) == 1440
+    assert _check_device_horizons(_fake_device_buffer((2, 720, 16384), np.float32), 2, True) == 720
+    with pytest.raises(InputError, match="fewer tiles"):
+        _check_device_horizons(_fake_device_buffer((2, 16384, 1440), np.float32), 3, False)
+    with pytest.raises(InputError, match="float32"):
+        _check_device_horizons(_fake_device_buffer((3, 16384, 1440), np.float64), 3, False)
+    with pytest.raises(InputError, match="16384 pixels"):
+        _check_device_horizons(_fake_device_buffer((3, 1440, 16384), np.float32), 3, False)
+    assert _check_device_vectors(_fake_device_buffer((40, 3), np.float64), "sun", 40) == 40
+    with pytest.raises(InputError, match="fewer entries"):
+        _check_device_vectors(_fake_device_buffer((39, 3), np.float64), "sun", 40)
+    with pytest.raises(InputError, match="float64"):
+        _check_device_vectors(_fake_device_buffer((40, 3), np.float32), "earth")
