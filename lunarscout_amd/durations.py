@@ -318,7 +318,7 @@ def _float_product(operation: str, *, dem, horizon_store, output_path, band_coun
                    descriptions, compute, observer_elevation_m, nodata, compress, overwrite,
                    backend, progress_callback, progress_event_callback, cancellation_requested,
                    verbose, device_id, validate=None, configuration=None,
-                   start_fresh: bool = False) -> Path:
+                   start_fresh: bool = False, timestamps_utc=()) -> Path:
     """Float32 multi-band product over the DEM's patches: ``compute(calculator, dem,
     tile, patch)`` yields the band blocks of one patch (the band order of the
     file); patches without a horizon tile are nodata. ``validate()`` runs after
@@ -354,9 +354,10 @@ def _float_product(operation: str, *, dem, horizon_store, output_path, band_coun
     progress = _Progress(operation, output, selected, verbose, progress_callback,
                          progress_event_callback)
     patches = enumerate_patches(dem.width, dem.height)
-    from .product_store import ResumableTiledProduct
+    from .product_store import ResumableTiledProduct, horizon_inventory_identity
 
     product = None
+    inventory = horizon_inventory_identity(store, patches, observer_elevation_m)
 
     def check_cancel():
         if cancellation_requested is not None and cancellation_requested():
@@ -378,6 +379,8 @@ def _float_product(operation: str, *, dem, horizon_store, output_path, band_coun
             affine_transform=dem.geo_transform, projection=dem.projection, nodata=nodata,
             compress=compress, descriptions=descriptions, algorithm=operation,
             configuration=dict(configuration, observer_elevation_m=float(observer_elevation_m)),
+            timestamps_utc=timestamps_utc, invalid_value=nodata,
+            horizon_inventory_identity=inventory, backend=selected,
             overwrite=overwrite, start_fresh=start_fresh)
         progress("start", len(product.completed), len(patches))
         for k, patch in enumerate(patches):
@@ -582,6 +585,7 @@ def _body_elevation_product(body: str, vectors_m, *, dem, horizon_store, output_
                           output_path=output_path, band_count=len(times),
                           descriptions=[str(t) for t in times], compute=compute,
                           validate=validate, observer_elevation_m=observer_elevation_m,
+                          timestamps_utc=times,
                           nodata=nodata, compress=compress, overwrite=overwrite, backend=backend,
                           progress_callback=progress_callback,
                           progress_event_callback=progress_event_callback,

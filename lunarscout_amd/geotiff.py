@@ -132,6 +132,35 @@ def _undo_predictor(block: np.ndarray, predictor: int, dtype: np.dtype, endian: 
     return values
 
 
+def _xml(text) -> str:
+    return (str(text).replace("&", "&amp;").replace("<", "&lt;").replace(">", "&gt;")
+            .replace('"', "&quot;"))
+
+
+def read_gdal_metadata(path) -> tuple[dict, dict]:
+    """GDAL metadata items of a GeoTIFF: ({name: value} of the dataset,
+    {band index (0-based): {name: value}}), descriptions included."""
+    import xml.etree.ElementTree as ET
+
+    path = Path(path).expanduser()
+    with open(path, "rb") as handle:
+        head = handle.read(4)
+    big = head in (b"II+\0", b"MM\0+")
+    endian = "<" if head[:2] == b"II" else ">"
+    data = np.memmap(path, dtype=np.uint8, mode="r")
+    tags = _read_ifd(data, endian, big)
+    del data
+    dataset, bands = {}, {}
+    if _GDAL_METADATA not in tags:
+        return dataset, bands
+    for item in ET.fromstring(tags[_GDAL_METADATA][0]).iter("Item"):
+        if "sample" in item.attrib:
+            bands.setdefault(int(item.attrib["sample"]), {})[item.attrib["name"]] = item.text or ""
+        else:
+            dataset[item.attrib["name"]] = item.text or ""
+    return dataset, bands
+
+
 def read_geotiff(path, band: int = 1) -> tuple[np.ndarray, GeoTiffInfo]:
     """One band (1-based) of a GeoTIFF or BigTIFF as its native dtype, with its georeferencing."""
     path = Path(path).expanduser()
@@ -321,10 +350,14 @@ class GeoTiffWriter:
     def __init__(self, path, width: int, height: int, bands: int, dtype, affine_transform,
                  projection: ProjectionParameters, *, nodata: float | None = None,
                  overwrite: bool = False, compress: bool = True,
-                 descriptions: list[str] | None = None, staging=None, resume=None):
+                 descriptions: list[str] | None = None, staging=None, resume=None,
+                 metadata: dict | None = None, band_metadata: list | None = None):
         """``staging``: a fixed staging file instead of a temporary one (the
         resumable product store); ``resume``: (offsets, counts, end) of the
-        blocks already in that file, which is truncated to ``end`` and appended to."""
+        blocks already in that file, which is truncated to ``end`` and appended to.
+        ``metadata`` / ``band_metadata`` (one dict per band): GDAL metadata items
+        of the dataset and of each band, written in the GDAL_METADATA tag as GDAL
+        keeps ``update_tags`` items (the reference tags its products so)."""
         self.path = Path(path).expanduser().resolve()
         self.dtype = np.dtype(dtype)
         if self.dtype == np.bool_ or self.dtype.kind not in "uif":
@@ -347,6 +380,9 @@ class GeoTiffWriter:
         self.offsets = np.zeros(blocks, dtype=np.uint64)
         self.counts = np.zeros(blocks, dtype=np.uint64)
         self.descriptions = descriptions
+        self.metadata = {str(k): str(v) for k, v in (metadata or {}).items()}
+        self.band_metadata = [{str(k): str(v) for k, v in (m or {}).items()}
+                              for m in (band_metadata or [])]
         self.path.parent.mkdir(parents=True, exist_ok=True)
         if staging is None:
             fd, tmp = tempfile.mkstemp(prefix=f".{self.path.name}.", suffix=".tmp.tif",
@@ -422,10 +458,13 @@ class GeoTiffWriter:
             tags.append((_TIEPOINT, 12, (0.0, 0.0, 0.0, t[0], t[3], 0.0)))
         if self.nodata is not None:
             tags.append((_GDAL_NODATA, 2, repr(float(self.nodata))))
-        if self.descriptions:
-            items = "".join(f'<Item name="DESCRIPTION" sample="{i}" role="description">{d}</Item>'
-                            for i, d in enumerate(self.descriptions))
-            tags.append((_GDAL_METADATA, 2, f"<GDALMetadata>{items}</GDALMetadata>"))
+        items = [f'<Item name="{_xml(k)}">{_xml(v)}</Item>' for k, v in self.metadata.items()]
+        for i, m in enumerate(self.band_metadata):
+            items += [f'<Item name="{_xml(k)}" sample="{i}">{_xml(v)}</Item>' for k, v in m.items()]
+        for i, d in enumerate(self.descriptions or []):
+            items.append(f'<Item name="DESCRIPTION" sample="{i}" role="description">{_xml(d)}</Item>')
+        if items:
+            tags.append((_GDAL_METADATA, 2, "<GDALMetadata>" + "".join(items) + "</GDALMetadata>"))
         return sorted(tags, key=lambda x: x[0])
 
     def close(self) -> Path:

@@ -1,22 +1,31 @@
 """Resumable tiled GeoTIFF products: staged file, job manifest, patch journal.
 
-Counterpart of the reference's ``ResumableTiledProduct``
+Counterpart of the reference's ``ResumableTiledProduct`` and ``ProductJob``
 (src/lunarscout/_numba_horizon/product_store.py:154-653) without rasterio:
 
 * the product is written to ``.<name>.lunarscout-partial.tif`` beside the
-  output, with ``.manifest.json`` (the job identity: size, dtype, bands,
-  nodata, compression, georeferencing, algorithm and configuration, and its
-  SHA-256 fingerprint) and ``.journal.jsonl`` beside it;
+  output, with ``.manifest.json`` and ``.journal.jsonl`` beside it;
+* the manifest holds the job identity with the reference's fields
+  (product_store.py:154-252): algorithm and version, size, dtype, band count,
+  the UTC timestamp of every band, per-band metadata, the invalid value,
+  nodata, compression, georeferencing, the configuration and the identity of
+  the horizon tiles the job reads (``horizon_inventory_identity``: path, size
+  and modification time of every patch's tile), and its SHA-256 fingerprint;
+  the backends that computed valid patches follow it as ``compute_backends``;
 * every finished patch appends one journal line -- its key, valid / invalid,
-  and the (offset, byte count) of each band block it wrote -- after the blocks
-  are on disk (fsync), so the journal never names data that is not there;
-* a run that finds all three files resumes: a different job identity raises
-  ``IncompatibleProductJobError`` (``start_fresh=True`` discards the staged
-  product); the staged file is cut back to the end of the last journaled
-  patch (a torn last line, or blocks written after it, are dropped) and the
-  completed patches are skipped;
-* ``finalize`` appends the TIFF directory, renames the staged file over the
-  output and removes the manifest and journal.
+  its backend and the (offset, byte count) of each band block it wrote --
+  after the blocks are on disk (fsync), so the journal never names data that
+  is not there;
+* a run that finds all three files resumes: a different job identity (a
+  different horizon store included) raises ``IncompatibleProductJobError``
+  (``start_fresh=True`` discards the staged product); the staged file is cut
+  back to the end of the last journaled patch (a torn last line, or blocks
+  written after it, are dropped) and the completed patches are skipped;
+* ``finalize`` appends the TIFF directory with the reference's GDAL metadata
+  tags (LUNARSCOUT_PRODUCT_SCHEMA, LUNARSCOUT_TIMESTAMPS_UTC,
+  LUNARSCOUT_COMPUTE_BACKENDS on the dataset, TIMESTAMP_UTC and the band
+  metadata on each band; product_store.py:27-32, 485-510, 624-640), renames the
+  staged file over the output and removes the manifest and journal.
 """
 
 from __future__ import annotations
@@ -24,6 +33,7 @@ from __future__ import annotations
 import hashlib
 import json
 import os
+from datetime import datetime
 from pathlib import Path
 from typing import Any
 
@@ -33,8 +43,13 @@ from .errors import ProductGenerationError
 from .geometry import projection_array
 from .geotiff import GeoTiffWriter
 
-MANIFEST_SCHEMA = "lunarscout-amd-product-job-v1"
+MANIFEST_SCHEMA = "lunarscout-amd-product-job-v2"
+ALGORITHM_VERSION = "lunarscout-amd-lighting-v2"
 PATCH_SIZE = 128
+SCHEMA_TAG = "LUNARSCOUT_PRODUCT_SCHEMA"
+TIMESTAMPS_TAG = "LUNARSCOUT_TIMESTAMPS_UTC"
+TIMESTAMP_TAG = "TIMESTAMP_UTC"
+COMPUTE_BACKENDS_TAG = "LUNARSCOUT_COMPUTE_BACKENDS"
 
 
 class IncompatibleProductJobError(ProductGenerationError):
@@ -61,12 +76,49 @@ def _nodata_json(value):
     return "NaN" if np.isnan(value) else value
 
 
+def utc_stamp(value) -> str:
+    """A band timestamp as the manifest and the tags hold it: UTC, microseconds, 'Z'."""
+    if isinstance(value, datetime):
+        stamp = value
+    else:
+        text = str(value).strip()
+        stamp = datetime.fromisoformat(text[:-1] + "+00:00" if text.endswith("Z") else text)
+    offset = stamp.utcoffset()
+    if offset is None:
+        raise ValueError("timestamps must include a UTC offset")
+    utc = (stamp - offset).replace(tzinfo=None)
+    return utc.strftime("%Y-%m-%dT%H:%M:%S.%fZ")
+
+
+def horizon_inventory_identity(store, patches, observer_elevation_m: float) -> str:
+    """Identity of the horizon tiles a product job reads.
+
+    One record per patch -- its tile's resolved path, byte size and
+    modification time, or a marker when the tile is missing -- hashed, so a
+    tile regenerated (or added, or removed) between a cancelled run and its
+    resume changes the job's fingerprint without re-reading any payload
+    (psr_pipeline.py:96-123 keys the reference's jobs the same way).
+    """
+    digest = hashlib.sha256()
+    for patch in patches:
+        path = store.find_existing_path(patch.tile_y, patch.tile_x, observer_elevation_m)
+        if path is None:
+            record = f"{patch.tile_y},{patch.tile_x},absent"
+        else:
+            st = Path(path).stat()
+            record = f"{patch.tile_y},{patch.tile_x},{Path(path).resolve()},{st.st_size},{st.st_mtime_ns}"
+        digest.update(record.encode("utf-8") + b"\n")
+    return "stat-sha256:" + digest.hexdigest()
+
+
 class ResumableTiledProduct:
     """A tiled, band-separate product written patch by patch, resumable."""
 
     def __init__(self, output_path, *, width: int, height: int, bands: int, dtype, affine_transform,
                  projection, nodata=None, compress: bool = True, descriptions=None,
                  algorithm: str = "unspecified", configuration: dict | None = None,
+                 timestamps_utc=(), band_metadata=(), invalid_value=0,
+                 horizon_inventory_identity: str = "unspecified", backend: str | None = None,
                  overwrite: bool = False, start_fresh: bool = False):
         self.output_path = Path(output_path).expanduser().resolve()
         self.staging_path = self.output_path.with_name(
@@ -77,19 +129,32 @@ class ResumableTiledProduct:
             raise ProductGenerationError("The product output already exists.",
                                          code="product_output_exists",
                                          details={"path": str(self.output_path)})
+        timestamps = [utc_stamp(t) for t in timestamps_utc]
+        if timestamps and len(timestamps) != int(bands):
+            raise ValueError("timestamp count must equal the band count")
+        band_metadata = [dict(m) for m in band_metadata] or [{} for _ in range(int(bands))]
+        if len(band_metadata) != int(bands):
+            raise ValueError("band metadata count must equal the band count")
         manifest = {
-            "schema": MANIFEST_SCHEMA, "algorithm": str(algorithm),
-            "width": int(width), "height": int(height), "bands": int(bands),
-            "dtype": np.dtype(dtype).str, "nodata": _nodata_json(nodata),
+            "schema": MANIFEST_SCHEMA, "algorithm_version": ALGORITHM_VERSION,
+            "algorithm": str(algorithm),
+            "width": int(width), "height": int(height), "dtype": np.dtype(dtype).str,
+            "band_count": int(bands), "timestamps_utc": timestamps,
+            "band_metadata": band_metadata, "invalid_value": _nodata_json(invalid_value),
+            "nodata": _nodata_json(nodata),
             "compression": "deflate" if compress else "none",
             "affine_transform": [float(v) for v in affine_transform],
             "projection": [float(v) for v in projection_array(projection)],
             "descriptions": list(descriptions or []),
             "configuration": configuration or {},
+            "horizon_inventory_identity": str(horizon_inventory_identity),
             "tile_size": PATCH_SIZE, "interleave": "band",
         }
         self.fingerprint = hashlib.sha256(_json_bytes(manifest)).hexdigest()
         manifest["fingerprint"] = self.fingerprint
+        self._manifest = manifest
+        self.backend = backend
+        self.compute_backends: list[str] = []
         self.completed: dict[str, str] = {}
         if start_fresh:
             self._remove_staging()
@@ -106,14 +171,22 @@ class ResumableTiledProduct:
                     "A staged product of a different job exists; use start_fresh=True.",
                     code="product_job_incompatible", details={"path": str(self.staging_path)})
             resume = self._read_journal(width, height, bands)
+        per_band = [{str(k): str(v) for k, v in m.items()} for m in band_metadata]
+        for i, stamp in enumerate(timestamps):
+            per_band[i][TIMESTAMP_TAG] = stamp
         self.writer = GeoTiffWriter(self.output_path, width, height, bands, dtype,
                                     affine_transform, projection, nodata=nodata,
                                     overwrite=True, compress=compress, descriptions=descriptions,
-                                    staging=self.staging_path, resume=resume)
+                                    staging=self.staging_path, resume=resume,
+                                    metadata={SCHEMA_TAG: MANIFEST_SCHEMA,
+                                              TIMESTAMPS_TAG: json.dumps(timestamps,
+                                                                         separators=(",", ":"))},
+                                    band_metadata=per_band)
         if resume is None:
             self.completed = {}
+            self.compute_backends = []
             self.writer.sync()
-            _atomic_json(self.manifest_path, manifest)
+            self._write_manifest()
             with open(self.journal_path, "wb") as handle:
                 handle.flush()
                 os.fsync(handle.fileno())
@@ -137,6 +210,9 @@ class ResumableTiledProduct:
             for index, offset, count in entry["blocks"]:
                 offsets[index], counts[index] = offset, count
             self.completed[entry["patch"]] = entry["state"]
+            backend = entry.get("backend")
+            if entry["state"] == "valid" and backend and backend not in self.compute_backends:
+                self.compute_backends.append(backend)
             end = max(end, int(entry["end"]))
             good += len(line)
         if good != len(raw):
@@ -145,6 +221,10 @@ class ResumableTiledProduct:
         if not self.completed:
             return None
         return offsets, counts, end
+
+    def _write_manifest(self) -> None:
+        _atomic_json(self.manifest_path, dict(self._manifest,
+                                              compute_backends=list(self.compute_backends)))
 
     def _remove_staging(self) -> None:
         for path in (self.staging_path, self.manifest_path, self.journal_path):
@@ -171,8 +251,13 @@ class ResumableTiledProduct:
             raise ValueError(f"a patch needs {self.writer.bands} bands, got {len(written)}")
         end = self.writer.sync()
         key = self.key(tile_y, tile_x)
-        self._journal.write(_json_bytes({"patch": key, "state": state, "end": end,
-                                         "blocks": written}) + b"\n")
+        entry = {"patch": key, "state": state, "end": end, "blocks": written}
+        if state == "valid" and self.backend:
+            entry["backend"] = self.backend
+            if self.backend not in self.compute_backends:
+                self.compute_backends.append(self.backend)
+                self._write_manifest()
+        self._journal.write(_json_bytes(entry) + b"\n")
         self._journal.flush()
         os.fsync(self._journal.fileno())
         self.completed[key] = state
@@ -185,6 +270,8 @@ class ResumableTiledProduct:
 
     def finalize(self) -> Path:
         self._journal.close()
+        self.writer.metadata[COMPUTE_BACKENDS_TAG] = json.dumps(self.compute_backends,
+                                                                separators=(",", ":"))
         path = self.writer.close()
         self.manifest_path.unlink(missing_ok=True)
         self.journal_path.unlink(missing_ok=True)

@@ -139,9 +139,10 @@ def _run(operation, dem_path, horizons_path, output_path, *, sun_vectors_m, time
                          progress_event_callback)
     store = HorizonTileStore(horizons)
     patches = enumerate_patches(dem.width, dem.height)
-    from .product_store import ResumableTiledProduct
+    from .product_store import ResumableTiledProduct, horizon_inventory_identity
 
     bands, prepared = bands_of(dem, vectors)
+    inventory = horizon_inventory_identity(store, patches, observer_height_m)
     product = None
 
     def check_cancel():
@@ -165,9 +166,10 @@ def _run(operation, dem_path, horizons_path, output_path, *, sun_vectors_m, time
                           if times is not None and len(times) == bands else None),
             algorithm=operation,
             configuration={"vectors_sha256": hashlib.sha256(vectors.tobytes()).hexdigest(),
-                           "invalid_value": int(invalid_value),
                            "observer_height_m": float(observer_height_m)},
-            overwrite=overwrite, start_fresh=start_fresh)
+            timestamps_utc=(times if times is not None and len(times) == bands else ()),
+            invalid_value=int(invalid_value), horizon_inventory_identity=inventory,
+            backend=selected, overwrite=overwrite, start_fresh=start_fresh)
         progress("start", len(product.completed), len(patches))
         for k, patch in enumerate(patches):
             if product.is_complete(patch.tile_y, patch.tile_x):

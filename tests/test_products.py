@@ -145,3 +145,51 @@ def test_products_resume_after_cancellation(site, tmp_path):
     a, _ = geotiff.read_geotiff(out, band=12)
     b, _ = geotiff.read_geotiff(fresh, band=12)
     np.testing.assert_array_equal(a, b)
+
+
+def test_manifest_fields_tags_and_horizon_identity(site, tmp_path):
+    """The job carries the reference's manifest fields and GeoTIFF tags
+    (product_store.py:27-32, 154-252, 485-510, 624-640), and a horizon tile
+    regenerated between a cancelled run and its resume changes the job."""
+    import json
+    import os
+    import shutil
+    from datetime import datetime, timedelta, timezone
+
+    from lunarscout_amd.product_store import IncompatibleProductJobError
+
+    dem, dem_path, horizons, written, sun = site
+    store_dir = tmp_path / "horizons"
+    shutil.copytree(horizons, store_dir)
+    times = [datetime(2027, 3, 1, tzinfo=timezone.utc) + timedelta(hours=6 * i) for i in range(6)]
+    calls = {"n": 0}
+
+    def cancel_in_second_patch():
+        calls["n"] += 1
+        return calls["n"] > 10
+
+    with pytest.raises(OperationCancelledError):
+        generate_lightmap(dem_path, store_dir, tmp_path / "m.tif", sun_vectors_m=sun[:6],
+                          times=times, backend="cpu", cancellation_requested=cancel_in_second_patch)
+    manifest = json.loads((tmp_path / ".m.tif.lunarscout-partial.manifest.json").read_text())
+    stamps = [t.strftime("%Y-%m-%dT%H:%M:%S.000000Z") for t in times]
+    assert manifest["timestamps_utc"] == stamps
+    assert manifest["band_count"] == 6 and len(manifest["band_metadata"]) == 6
+    assert manifest["invalid_value"] == 0 and manifest["compute_backends"] == ["cpu"]
+    assert manifest["horizon_inventory_identity"].startswith("stat-sha256:")
+    # rewrite one horizon tile (same values, new file): the staged job no longer matches
+    tile = next(store_dir.rglob("horizon_00000_00128_000.cbin"))
+    payload = tile.read_bytes()
+    tile.unlink()
+    tile.write_bytes(payload)
+    os.utime(tile, ns=(tile.stat().st_atime_ns, tile.stat().st_mtime_ns + 10**9))
+    with pytest.raises(IncompatibleProductJobError):
+        generate_lightmap(dem_path, store_dir, tmp_path / "m.tif", sun_vectors_m=sun[:6],
+                          times=times, backend="cpu")
+    out = generate_lightmap(dem_path, store_dir, tmp_path / "m.tif", sun_vectors_m=sun[:6],
+                            times=times, backend="cpu", start_fresh=True)
+    dataset, bands = geotiff.read_gdal_metadata(out)
+    assert dataset["LUNARSCOUT_PRODUCT_SCHEMA"].startswith("lunarscout-")
+    assert json.loads(dataset["LUNARSCOUT_TIMESTAMPS_UTC"]) == stamps
+    assert json.loads(dataset["LUNARSCOUT_COMPUTE_BACKENDS"]) == ["cpu"]
+    assert [bands[i]["TIMESTAMP_UTC"] for i in range(6)] == stamps
