@@ -227,6 +227,11 @@ int lsh_batch_create(lsh_session* session, const lsh_batch_site* sites, int32_t 
                      int32_t max_tiles, int32_t azimuth_count, double maximum_distance_m,
                      lsh_batch** batch);
 
+/* The workgroup count at which every workgroup of a batch kernel shape is
+ * resident at once on the session's device (the kernel's occupancy per CU x the
+ * CU count, a multiple of 8): what lsh_batch_run uses when workgroups <= 0. */
+int lsh_batch_resident_workgroups(lsh_session* session, int32_t config, int32_t* workgroups);
+
 /* Horizons of tile_count <= max_tiles tiles into output (device float32
  * [tile_count][16384][azimuth_count], pixel-major per tile), in one persistent
  * launch: `workgroups` resident workgroups (a multiple of 8; <= 0 picks one

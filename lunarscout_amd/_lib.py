@@ -126,6 +126,7 @@ SIGNATURES = {
     "lsh_horizon_adjoint": [_vp, _vp, _i32, _vp, _i32, _i32, _i32, _f32, _vp, _vp, _vp, _f64, _vp],
     "lsh_batch_shape_count": [ctypes.POINTER(_i32)],
     "lsh_batch_shape": [_i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
+    "lsh_batch_resident_workgroups": [_vp, _i32, ctypes.POINTER(_i32)],
     "lsh_batch_create": [_vp, _vp, _i32, _i32, _i32, _f64, ctypes.POINTER(_vp)],
     "lsh_batch_run": [_vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp],
     "lsh_batch_destroy": [_vp],

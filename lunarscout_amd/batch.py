@@ -141,14 +141,13 @@ def autotune(batch: HorizonBatch, sample: Sequence[tuple[int, int, int]], output
         except (OSError, ValueError):
             pass
     lib = session.lib
-    cus = ctypes.c_int32()
-    name = ctypes.create_string_buffer(256)
-    check(lib.lsh_device_info(session.device_id, name, len(name), ctypes.byref(cus)),
-          "lsh_device_info")
     trials = []
     for config in (configs if configs is not None else range(len(shapes()))):
         bx, by, az = shapes()[config]
-        resident = max(8, int(cus.value) * max(1, 16 // az))  # 16 waves per CU
+        count = ctypes.c_int32()
+        check(lib.lsh_batch_resident_workgroups(session.handle, config, ctypes.byref(count)),
+              "lsh_batch_resident_workgroups")
+        resident = max(8, int(count.value))  # every workgroup resident at the kernel's occupancy
         for factor in workgroup_factors:
             wgs = max(8, int(resident * factor) // 8 * 8)
             e0 = Event(session).record()

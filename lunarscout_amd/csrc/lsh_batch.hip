@@ -109,6 +109,15 @@ int lsh_batch_shape(int32_t config, int32_t* block_x, int32_t* block_y, int32_t*
   return LSH_OK;
 }
 
+int lsh_batch_resident_workgroups(lsh_session* session, int32_t config, int32_t* workgroups) {
+  lsh::Session* s = reinterpret_cast<lsh::Session*>(session);
+  LSH_REQUIRE(s && workgroups, "invalid arguments");
+  LSH_REQUIRE(config >= 0 && config < lsh::batch_shape_count(), "unknown kernel shape");
+  if (int r = lsh::select_device(s)) return r;
+  *workgroups = lsh::batch_default_workgroups(config, s->cu_count);
+  return LSH_OK;
+}
+
 int lsh_batch_create(lsh_session* session, const lsh_batch_site* sites, int32_t site_count,
                      int32_t max_tiles, int32_t azimuth_count, double maximum_distance_m,
                      lsh_batch** batch) {

@@ -68,6 +68,9 @@ struct PassSegment {
   S sx, sy;
   S a1, a2, a3, a4, b1, b2, b3, b4;
   S s_start, s_end, s_chord, c1, c2, c3;
+  // tangent coefficients 2 a2, 3 a3, 4 a4 (and b), the products tangent()
+  // forms first (cuda_backend.py:73-84)
+  S ta2, ta3, ta4, tb2, tb3, tb4;
 };
 
 struct MarchCounters {
@@ -100,21 +103,23 @@ LSH_HD S lsh_fma(S a, S b, S c) {
                         : static_cast<S>(__builtin_fma(double(a), double(b), double(c)));
 }
 
-// evaluate and tangent (cuda_backend.py:62-84) at the start offset d, in one
-// Horner pass with fused multiply-adds: the partial sums b3 = a4 d + a3,
-// b2 = b3 d + a2, b1 = b2 d + a1 give the position b1 d + x0, and the same
-// recurrence on them gives the derivative 4 a4 d^3 + 3 a3 d^2 + 2 a2 d + a1
-// (c3 = a4 d + b3, c2 = c3 d + b2, c2 d + b1) -- no per-ray 2 a2, 3 a3, 4 a4.
+// evaluate (cuda_backend.py:62-71) on the start offset, in Horner form with
+// fused multiply-adds: x0 + d (a1 + d (a2 + d (a3 + d a4))).
 template <class S>
-LSH_HD void lsh_quartic_and_tangent(S x0, S a1, S a2, S a3, S a4, S d, S& position,
-                                    S& tangent) {
-  const S b3 = lsh_fma(d, a4, a3);
-  const S b2 = lsh_fma(d, b3, a2);
-  const S b1 = lsh_fma(d, b2, a1);
-  position = lsh_fma(d, b1, x0);
-  const S c3 = lsh_fma(d, a4, b3);
-  const S c2 = lsh_fma(d, c3, b2);
-  tangent = lsh_fma(d, c2, b1);
+LSH_HD S lsh_quartic(S x0, S a1, S a2, S a3, S a4, S d) {
+  S r = lsh_fma(d, a4, a3);
+  r = lsh_fma(d, r, a2);
+  r = lsh_fma(d, r, a1);
+  return lsh_fma(d, r, x0);
+}
+
+// tangent (cuda_backend.py:73-84) in Horner form: a1 + d (2 a2 + d (3 a3 + d 4 a4)),
+// with t2 = 2 a2, t3 = 3 a3, t4 = 4 a4 formed once per ray.
+template <class S>
+LSH_HD S lsh_tangent(S a1, S t2, S t3, S t4, S d) {
+  S r = lsh_fma(d, t4, t3);
+  r = lsh_fma(d, r, t2);
+  return lsh_fma(d, r, a1);
 }
 
 template <class T>
@@ -294,9 +299,8 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   while (s.at_most(g.s_end)) {
     // -- position (cuda_backend.py:672-705)
     const S delta = s.minus(g.s_start);
-    S px, py, dx, dy;
-    lsh_quartic_and_tangent(g.sx, g.a1, g.a2, g.a3, g.a4, delta, px, dx);
-    lsh_quartic_and_tangent(g.sy, g.b1, g.b2, g.b3, g.b4, delta, py, dy);
+    const S px = lsh_quartic(g.sx, g.a1, g.a2, g.a3, g.a4, delta);
+    const S py = lsh_quartic(g.sy, g.b1, g.b2, g.b3, g.b4, delta);
     // the reference's px in [0, W-1), py in [0, H-1), tested on the absolute cell
     // index floor(px) + bx (exact: integers); NaN fails every comparison
     const S fx = lsh_floor(px), fy = lsh_floor(py);
@@ -320,6 +324,8 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     int level = (lsh_exponent(footprint) - d.resolution_exponent + 1) >> 1;
     if (!(lsh_ldexp(res, 2 * level) <= footprint)) --level;
     level = footprint >= res ? (level < top_level ? level : top_level) : 0;
+    const S dx = lsh_tangent(g.a1, g.ta2, g.ta3, g.ta4, delta);
+    const S dy = lsh_tangent(g.b1, g.tb2, g.tb3, g.tb4, delta);
     // -- hierarchy tests at this position (cuda_backend.py:706-868)
     // Each level's outcome -- the ray leaves the level (it ends), a nodata
     // block, a cull, a level-0 sample, or a descent -- is formed without
@@ -521,6 +527,8 @@ LSH_HD PassSegment<S> lsh_interpolate(const float* p00, const float* p10, const 
   g.s_start = LSH_FIELD(12); g.s_end = LSH_FIELD(13); g.s_chord = LSH_FIELD(14);
   g.c1 = LSH_FIELD(15); g.c2 = LSH_FIELD(16); g.c3 = LSH_FIELD(17);
 #undef LSH_FIELD
+  g.ta2 = S(2) * g.a2; g.ta3 = S(3) * g.a3; g.ta4 = S(4) * g.a4;
+  g.tb2 = S(2) * g.b2; g.tb3 = S(3) * g.b3; g.tb4 = S(4) * g.b4;
   return g;
 }
 
