@@ -86,10 +86,12 @@ class HorizonBatch:
         self.handle = handle.value
 
     def run(self, tiles: Sequence[tuple[int, int, int]], output: DeviceBuffer, *, config: int = 0,
-            workgroups: int = 0, degrees: bool = False, use_graph: bool = True) -> None:
+            workgroups: int = 0, degrees: bool = False, use_graph: bool = True,
+            azimuth_major: bool = False) -> None:
         """March ``tiles`` = [(site, tile_column, tile_row), ...] into ``output``.
 
-        ``output``: device float32 [>= len(tiles), 16384, N]. Stream ordered.
+        ``output``: device float32 [>= len(tiles), 16384, N], or [.., N, 16384]
+        with ``azimuth_major``. Stream ordered.
         """
         n = len(tiles)
         if output.nbytes < n * 16384 * self.azimuth_count * 4:
@@ -99,7 +101,8 @@ class HorizonBatch:
             table[i].site, table[i].tile_column, table[i].tile_row = int(site), int(column), int(row)
         check(self.session.lib.lsh_batch_run(self.handle, ctypes.addressof(table), n, int(config),
                                              int(workgroups),
-                                             _lib.HORIZON_DEGREES if degrees else 0,
+                                             (_lib.HORIZON_DEGREES if degrees else 0)
+                                             | (_lib.HORIZON_AZIMUTH_MAJOR if azimuth_major else 0),
                                              1 if use_graph else 0, output.ptr), "lsh_batch_run")
 
     def close(self) -> None:

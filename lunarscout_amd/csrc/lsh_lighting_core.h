@@ -185,28 +185,39 @@ LSH_LHD float sun_fraction(const Row& hz, int N, float az_deg, float el_deg,
 }
 
 
-// Every output of one pixel of one tile: the time series walked in order.
+// The three independent series of one pixel: the Sun walk (per-time fractions
+// and their mean), the Earth margins (and the visible fraction) and the PSR
+// limb test. Each writes only its own outputs, so they may run as separate
+// work items.
+constexpr unsigned kSunSeries = 1u, kEarthSeries = 2u, kPsrSeries = 4u;
+constexpr unsigned kAllSeries = kSunSeries | kEarthSeries | kPsrSeries;
+
+// The outputs of one pixel of one tile for the series in `series`: each time
+// series walked in order.
 template <bool kAzimuthMajor>
-LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origin) {
+LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origin,
+                         unsigned series = kAllSeries) {
   const int line = pixel / 128, sample = pixel % 128;
   const int row = origin.tile_row + line, column = origin.tile_column + sample;
   const int64_t tile_pixels = static_cast<int64_t>(tile) * kPatchPixels;
   const bool inside = row < a.dem_height && column < a.dem_width;
+  const bool sun = (series & kSunSeries) != 0, earth = (series & kEarthSeries) != 0;
+  const bool limb = (series & kPsrSeries) != 0;
   if (!inside) {  // edge padding (psr_cuda.py:74-76)
-    if (a.sun_bytes)
+    if (sun && a.sun_bytes)
       for (int t = 0; t < a.sun_count; ++t)
         a.sun_bytes[(static_cast<int64_t>(tile) * a.sun_count + t) * kPatchPixels + pixel] = 0;
-    if (a.sun_fraction)
+    if (sun && a.sun_fraction)
       for (int t = 0; t < a.sun_count; ++t)
         a.sun_fraction[(static_cast<int64_t>(tile) * a.sun_count + t) * kPatchPixels + pixel] = 0.0f;
-    if (a.earth_margin)
+    if (earth && a.earth_margin)
       for (int t = 0; t < a.earth_count; ++t)
         a.earth_margin[(static_cast<int64_t>(tile) * a.earth_count + t) * kPatchPixels + pixel] =
             -INFINITY;
-    if (a.sun_mean) a.sun_mean[tile_pixels + pixel] = 0.0f;
-    if (a.earth_visible) a.earth_visible[tile_pixels + pixel] = 0.0f;
-    if (a.psr) a.psr[tile_pixels + pixel] = 0;
-    if (a.psr_margin) a.psr_margin[tile_pixels + pixel] = -INFINITY;
+    if (sun && a.sun_mean) a.sun_mean[tile_pixels + pixel] = 0.0f;
+    if (earth && a.earth_visible) a.earth_visible[tile_pixels + pixel] = 0.0f;
+    if (limb && a.psr) a.psr[tile_pixels + pixel] = 0;
+    if (limb && a.psr_margin) a.psr_margin[tile_pixels + pixel] = -INFINITY;
     return;
   }
   const int N = a.N;
@@ -216,7 +227,7 @@ LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origi
                           : a.horizons + (tile_pixels + pixel) * N;
   const Frame f = pixel_frame(a, row, column);
 
-  if (a.sun_count > 0 && (a.sun_bytes || a.sun_fraction || a.sun_mean)) {
+  if (sun && a.sun_count > 0 && (a.sun_bytes || a.sun_fraction || a.sun_mean)) {
     float acc = 0.0f;
     for (int t = 0; t < a.sun_count; ++t) {
       double az, el;
@@ -229,7 +240,7 @@ LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origi
     }
     if (a.sun_mean) a.sun_mean[tile_pixels + pixel] = acc / static_cast<float>(a.sun_count);
   }
-  if (a.earth_count > 0 && (a.earth_margin || a.earth_visible)) {
+  if (earth && a.earth_count > 0 && (a.earth_margin || a.earth_visible)) {
     int visible = 0;
     for (int t = 0; t < a.earth_count; ++t) {
       double az, el;
@@ -243,7 +254,7 @@ LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origi
     if (a.earth_visible)
       a.earth_visible[tile_pixels + pixel] = static_cast<float>(visible) / a.earth_count;
   }
-  if (a.psr_count > 0 && (a.psr || a.psr_margin)) {
+  if (limb && a.psr_count > 0 && (a.psr || a.psr_margin)) {
     // the mask alone is decided by the first vector that lights the limb
     // (psr.py:136-139); the margin needs every vector
     const bool mask_only = a.psr_margin == nullptr;

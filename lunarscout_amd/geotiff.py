@@ -401,18 +401,27 @@ class GeoTiffWriter:
             self.handle.truncate(int(end))
             self.handle.seek(int(end))
 
-    def write_block(self, band: int, by: int, bx: int, block: np.ndarray) -> None:
-        """One 128x128 block (smaller edge blocks are zero-padded) of a 0-based band."""
+    def encode_block(self, block: np.ndarray) -> bytes:
+        """The stored bytes of one block (smaller edge blocks are zero-padded to
+        128x128). Touches no writer state, so blocks may be encoded on several
+        threads (zlib releases the GIL) and written in order afterwards."""
         tile = np.zeros((self.TILE, self.TILE), dtype=self.dtype)
         part = np.asarray(block, dtype=self.dtype)
         tile[: part.shape[0], : part.shape[1]] = part
-        payload = _encode_block(tile, self.predictor, self.compress)
+        return _encode_block(tile, self.predictor, self.compress)
+
+    def write_encoded(self, band: int, by: int, bx: int, payload: bytes) -> None:
+        """Append a block's stored bytes (``encode_block``) and record its place."""
         index = (band * self.down + by) * self.across + bx
         self.offsets[index] = self.handle.tell()
         self.counts[index] = len(payload)
         self.handle.write(payload)
         if len(payload) % 2:
             self.handle.write(b"\0")
+
+    def write_block(self, band: int, by: int, bx: int, block: np.ndarray) -> None:
+        """One 128x128 block (smaller edge blocks are zero-padded) of a 0-based band."""
+        self.write_encoded(band, by, bx, self.encode_block(block))
 
     def block_index(self, band: int, by: int, bx: int) -> int:
         return (band * self.down + by) * self.across + bx

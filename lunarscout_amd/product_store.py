@@ -15,7 +15,14 @@ Counterpart of the reference's ``ResumableTiledProduct`` and ``ProductJob``
 * every finished patch appends one journal line -- its key, valid / invalid,
   its backend and the (offset, byte count) of each band block it wrote --
   after the blocks are on disk (fsync), so the journal never names data that
-  is not there;
+  is not there. ``write_patch`` makes each patch durable by itself;
+  ``batch_writer(n)`` writes patches back to back and makes them durable n at a
+  time (one fsync of the staged file and one of the journal per n patches --
+  the reference's ``ProductBatchWriter``, product_store.py:655-794), encoding
+  the band blocks on a thread pool; a crash loses at most the patches since
+  the last checkpoint, which a resume computes again;
+* ``output_transform`` (product_store.py:362-385) maps every valid band block
+  before it is stored and must keep its shape and return the product's dtype;
 * a run that finds all three files resumes: a different job identity (a
   different horizon store included) raises ``IncompatibleProductJobError``
   (``start_fresh=True`` discards the staged product); the staged file is cut
@@ -33,9 +40,10 @@ from __future__ import annotations
 import hashlib
 import json
 import os
+from concurrent.futures import ThreadPoolExecutor
 from datetime import datetime
 from pathlib import Path
-from typing import Any
+from typing import Any, Callable
 
 import numpy as np
 
@@ -119,7 +127,8 @@ class ResumableTiledProduct:
                  algorithm: str = "unspecified", configuration: dict | None = None,
                  timestamps_utc=(), band_metadata=(), invalid_value=0,
                  horizon_inventory_identity: str = "unspecified", backend: str | None = None,
-                 overwrite: bool = False, start_fresh: bool = False):
+                 overwrite: bool = False, start_fresh: bool = False,
+                 output_transform: Callable[[np.ndarray], np.ndarray] | None = None):
         self.output_path = Path(output_path).expanduser().resolve()
         self.staging_path = self.output_path.with_name(
             f".{self.output_path.name}.lunarscout-partial.tif")
@@ -153,6 +162,7 @@ class ResumableTiledProduct:
         self.fingerprint = hashlib.sha256(_json_bytes(manifest)).hexdigest()
         manifest["fingerprint"] = self.fingerprint
         self._manifest = manifest
+        self.output_transform = output_transform
         self.backend = backend
         self.compute_backends: list[str] = []
         self.completed: dict[str, str] = {}
@@ -238,35 +248,76 @@ class ResumableTiledProduct:
     def is_complete(self, tile_y: int, tile_x: int) -> bool:
         return self.key(tile_y, tile_x) in self.completed
 
-    def write_patch(self, tile_y: int, tile_x: int, blocks, *, state: str = "valid") -> None:
-        """All bands of one patch (``blocks``: iterable of [h, w] arrays in band
-        order), then its journal line."""
-        by, bx = tile_y // PATCH_SIZE, tile_x // PATCH_SIZE
-        written = []
-        for band, block in enumerate(blocks):
-            index = self.writer.block_index(band, by, bx)
-            self.writer.write_block(band, by, bx, block)
-            written.append([index, int(self.writer.offsets[index]), int(self.writer.counts[index])])
-        if len(written) != self.writer.bands:
-            raise ValueError(f"a patch needs {self.writer.bands} bands, got {len(written)}")
-        end = self.writer.sync()
+    def _prepare(self, block, state: str) -> np.ndarray:
+        block = np.asarray(block)
+        if state != "valid" or self.output_transform is None:
+            return block
+        mapped = np.asarray(self.output_transform(block))
+        if mapped.shape != block.shape:
+            raise ValueError(f"output_transform must keep the block shape {block.shape}, "
+                             f"got {mapped.shape}")
+        if mapped.dtype != self.writer.dtype:
+            raise ValueError(f"output_transform returned {mapped.dtype}; the product holds "
+                             f"{self.writer.dtype}")
+        return mapped
+
+    def _store_patch(self, tile_y: int, tile_x: int, blocks, state: str, encode=None) -> dict:
+        """Write every band block of a patch; return its (not yet durable) journal entry."""
         key = self.key(tile_y, tile_x)
-        entry = {"patch": key, "state": state, "end": end, "blocks": written}
+        if key in self.completed:
+            raise ValueError(f"patch {key} is already complete")
+        by, bx = tile_y // PATCH_SIZE, tile_x // PATCH_SIZE
+        prepared = [self._prepare(block, state) for block in blocks]
+        if len(prepared) != self.writer.bands:
+            raise ValueError(f"a patch needs {self.writer.bands} bands, got {len(prepared)}")
+        if encode is None:
+            payloads = [self.writer.encode_block(b) for b in prepared]
+        else:
+            payloads = list(encode(self.writer.encode_block, prepared))
+        written = []
+        for band, payload in enumerate(payloads):
+            index = self.writer.block_index(band, by, bx)
+            self.writer.write_encoded(band, by, bx, payload)
+            written.append([index, int(self.writer.offsets[index]), int(self.writer.counts[index])])
+        entry = {"patch": key, "state": state, "blocks": written}
         if state == "valid" and self.backend:
             entry["backend"] = self.backend
-            if self.backend not in self.compute_backends:
-                self.compute_backends.append(self.backend)
-                self._write_manifest()
-        self._journal.write(_json_bytes(entry) + b"\n")
+        return entry
+
+    def _checkpoint(self, entries: list[dict]) -> list[str]:
+        """Make the written patches durable: the staged blocks first, then their
+        journal lines. Returns their keys."""
+        if not entries:
+            return []
+        end = self.writer.sync()
+        fresh = [e["backend"] for e in entries
+                 if e.get("backend") and e["backend"] not in self.compute_backends]
+        for backend in fresh:
+            if backend not in self.compute_backends:
+                self.compute_backends.append(backend)
+        if fresh:
+            self._write_manifest()
+        self._journal.write(b"".join(_json_bytes(dict(e, end=end)) + b"\n" for e in entries))
         self._journal.flush()
         os.fsync(self._journal.fileno())
-        self.completed[key] = state
+        for e in entries:
+            self.completed[e["patch"]] = e["state"]
+        return [e["patch"] for e in entries]
+
+    def write_patch(self, tile_y: int, tile_x: int, blocks, *, state: str = "valid") -> None:
+        """All bands of one patch (``blocks``: iterable of [h, w] arrays in band
+        order), made durable with its journal line before returning."""
+        self._checkpoint([self._store_patch(tile_y, tile_x, blocks, state)])
 
     def write_invalid_patch(self, tile_y: int, tile_x: int, height: int, width: int,
                             fill) -> None:
         block = np.full((height, width), fill, dtype=self.writer.dtype)
-        self.write_patch(tile_y, tile_x, (block for _ in range(self.writer.bands)),
-                         state="invalid")
+        self.write_patch(tile_y, tile_x, [block] * self.writer.bands, state="invalid")
+
+    def batch_writer(self, checkpoint_patch_count: int, *,
+                     encode_workers: int = 0) -> "ProductBatchWriter":
+        """A writer that checkpoints every ``checkpoint_patch_count`` patches."""
+        return ProductBatchWriter(self, checkpoint_patch_count, encode_workers=encode_workers)
 
     def finalize(self) -> Path:
         self._journal.close()
@@ -281,3 +332,84 @@ class ResumableTiledProduct:
         """Stop writing and keep the staged product for a later resume."""
         self._journal.close()
         self.writer.handle.close()
+
+
+class ProductBatchWriter:
+    """Patches written back to back, made durable ``checkpoint_patch_count`` at a
+    time (product_store.py:655-794). ``write_patch`` and ``checkpoint`` return
+    the keys of the patches a checkpoint made durable (empty when none did);
+    leaving the ``with`` block checkpoints what was written, also on an error
+    (every written patch is whole: a patch enters the batch after its last
+    band). ``encode_workers`` > 1 compresses a patch's band blocks on that many
+    threads."""
+
+    def __init__(self, product: ResumableTiledProduct, checkpoint_patch_count: int, *,
+                 encode_workers: int = 0):
+        if isinstance(checkpoint_patch_count, bool) or not isinstance(checkpoint_patch_count, int) \
+                or checkpoint_patch_count < 1:
+            raise ValueError("checkpoint_patch_count must be a positive integer")
+        self.product = product
+        self.limit = checkpoint_patch_count
+        self.pending: list[dict] = []
+        self.pool = ThreadPoolExecutor(encode_workers, thread_name_prefix="product-encode") \
+            if encode_workers > 1 else None
+        self.invalid_payloads: dict[tuple, bytes] = {}
+        self.closed = False
+
+    def _encode(self, fn, blocks):
+        return self.pool.map(fn, blocks) if self.pool is not None else map(fn, blocks)
+
+    def write_patch(self, tile_y: int, tile_x: int, blocks, *, state: str = "valid") -> list[str]:
+        if self.closed:
+            raise ValueError("the batch writer is closed")
+        key = self.product.key(tile_y, tile_x)
+        if any(e["patch"] == key for e in self.pending):
+            raise ValueError(f"patch {key} is already written")
+        self.pending.append(self.product._store_patch(tile_y, tile_x, blocks, state, self._encode))
+        return self.checkpoint() if len(self.pending) >= self.limit else []
+
+    def write_invalid_patch(self, tile_y: int, tile_x: int, height: int, width: int,
+                            fill) -> list[str]:
+        """Every band holds ``fill``: one block encoded once for all bands."""
+        writer = self.product.writer
+        shape = (height, width, fill)
+        payload = self.invalid_payloads.get(shape)
+        if payload is None:
+            payload = writer.encode_block(np.full((height, width), fill, dtype=writer.dtype))
+            self.invalid_payloads[shape] = payload
+        return self._write_encoded_patch(tile_y, tile_x, [payload] * writer.bands)
+
+    def _write_encoded_patch(self, tile_y: int, tile_x: int, payloads: list[bytes]) -> list[str]:
+        product, writer = self.product, self.product.writer
+        key = product.key(tile_y, tile_x)
+        if key in product.completed or any(e["patch"] == key for e in self.pending):
+            raise ValueError(f"patch {key} is already written")
+        by, bx = tile_y // PATCH_SIZE, tile_x // PATCH_SIZE
+        written = []
+        for band, payload in enumerate(payloads):
+            index = writer.block_index(band, by, bx)
+            writer.write_encoded(band, by, bx, payload)
+            written.append([index, int(writer.offsets[index]), int(writer.counts[index])])
+        self.pending.append({"patch": key, "state": "invalid", "blocks": written})
+        return self.checkpoint() if len(self.pending) >= self.limit else []
+
+    def checkpoint(self) -> list[str]:
+        entries, self.pending = self.pending, []
+        return self.product._checkpoint(entries)
+
+    def close(self) -> list[str]:
+        if self.closed:
+            return []
+        self.closed = True
+        try:
+            return self.checkpoint()
+        finally:
+            if self.pool is not None:
+                self.pool.shutdown(wait=True)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        self.close()
+        return False

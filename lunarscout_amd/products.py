@@ -14,13 +14,26 @@ Drop-ins for the reference's ``generate_lightmap`` and ``generate_psr``
   a staged file and an atomic rename: the lightmap holds one uint8 band per
   time, trunc(255 * visible fraction) of the 16-slice 0.27-degree Sun disk; the
   PSR mask one uint8 band (255 = permanently shadowed). Pixels of patches
-  whose horizon tile is missing carry ``invalid_value``;
-* progress events and cancellation between patches.
+  whose horizon tile is missing or unreadable carry ``invalid_value``;
+  ``output_transform`` / ``output_dtype`` / ``output_transform_id`` convert
+  valid patches before they are stored (products.py:361-372);
+* ``pipeline_mode="bounded"`` (the default) overlaps tile reads and decodes
+  (``reader_worker_count`` threads, at most ``decoded_horizon_capacity``
+  decoded tiles alive), the calculation on the calling thread and the writer
+  thread (``writer_queue_capacity`` computed patches queued for it), which
+  compresses band blocks on a thread pool and makes patches durable
+  ``durable_batch_size`` at a time (psr_pipeline.py:126-160, 523-797);
+  ``"serial"`` runs the same steps one patch after another;
+* progress events -- start, read, calculate, write, then valid / invalid when
+  a patch is durable, complete -- and cancellation between patches and bands.
 """
 
 from __future__ import annotations
 
+import os
+import queue
 import sys
+import threading
 from pathlib import Path
 from typing import Callable
 
@@ -99,8 +112,14 @@ class _Progress:
         self.progress_callback, self.event_callback = progress_callback, event_callback
         self.last = None
         self.callback_error: BaseException | None = None
+        self.completed = 0  # durable patches, advanced by the writer
+        self.lock = threading.Lock()  # the pipeline's stages report from their threads
 
     def __call__(self, stage, completed, total, tile_y=None, tile_x=None):
+        with self.lock:
+            self._emit(stage, completed, total, tile_y, tile_x)
+
+    def _emit(self, stage, completed, total, tile_y, tile_x):
         fraction = 1.0 if total == 0 else completed / total
         event = ProgressEvent(self.operation, stage, completed, total, fraction, self.backend,
                               f"{self.operation} {stage}", tile_y, tile_x, self.output)
@@ -120,13 +139,178 @@ class _Progress:
             raise
 
 
+def _validate_output_conversion(output_transform, output_dtype, output_transform_id):
+    """products.py:36-70: the optional per-patch conversion of valid pixels."""
+    if output_transform is None:
+        if output_dtype is not None or output_transform_id is not None:
+            raise InputError("output_dtype and output_transform_id require output_transform.",
+                             code="product_output_conversion_invalid")
+        return np.dtype(np.uint8)
+    if not callable(output_transform):
+        raise InputError("output_transform must be callable or None.",
+                         code="product_output_conversion_invalid")
+    if output_dtype is None:
+        raise InputError("output_dtype is required with output_transform.",
+                         code="product_output_conversion_invalid")
+    try:
+        dtype = np.dtype(output_dtype)
+    except (TypeError, ValueError) as exc:
+        raise InputError("output_dtype is not a valid NumPy dtype.",
+                         code="product_output_dtype_invalid",
+                         details={"output_dtype": repr(output_dtype)}) from exc
+    if output_transform_id is not None and not isinstance(output_transform_id, str):
+        raise InputError("output_transform_id must be a string or None.",
+                         code="product_output_conversion_invalid")
+    return dtype
+
+
+def _validate_pipeline(mode, reader_workers, decoded_capacity, writer_capacity, batch):
+    """The bounded pipeline's knobs (psr_pipeline.py:126-160)."""
+    if mode not in ("serial", "bounded"):
+        raise InputError("pipeline_mode must be 'serial' or 'bounded'.",
+                         code="product_pipeline_invalid", details={"pipeline_mode": mode})
+    for name, value in (("reader_worker_count", reader_workers),
+                        ("decoded_horizon_capacity", decoded_capacity),
+                        ("writer_queue_capacity", writer_capacity),
+                        ("durable_batch_size", batch)):
+        if isinstance(value, bool) or not isinstance(value, (int, np.integer)) or value < 1:
+            raise InputError(f"{name} must be a positive integer.",
+                             code="product_pipeline_invalid", details={name: value})
+
+
+class _Stop(Exception):
+    """Raised inside a stage when another stage has already failed."""
+
+
+class _Overlap:
+    """Read -> compute -> write over the pending patches, bounded.
+
+    Horizon tiles are read and decoded on ``reader_workers`` threads (the codec
+    runs in C and releases the GIL), at most ``decoded_capacity`` decoded tiles
+    alive at once, handed over in patch order; the calling thread computes each
+    patch (the GPU launches stay on one thread); one writer thread encodes and
+    stores the band blocks and checkpoints them ``durable_batch_size`` at a time
+    (psr_pipeline.py:523-797 runs its reader, CUDA and writer stages so). The
+    first error of any stage stops the others and is raised by ``run``.
+    """
+
+    def __init__(self, pending, read, compute, write, finish, *, reader_workers, decoded_capacity,
+                 writer_capacity):
+        self.pending, self.read, self.compute = pending, read, compute
+        self.write, self.finish = write, finish
+        self.reader_workers = reader_workers
+        self.slots = threading.BoundedSemaphore(decoded_capacity)
+        self.decoded: queue.Queue = queue.Queue()
+        self.computed: queue.Queue = queue.Queue(maxsize=writer_capacity)
+        self.stop = threading.Event()
+        self.errors: list[BaseException] = []
+        self.lock = threading.Lock()
+
+    def fail(self, error: BaseException) -> None:
+        with self.lock:
+            if not self.errors:
+                self.errors.append(error)
+        self.stop.set()
+
+    def _put(self, q: queue.Queue, item) -> None:
+        while True:
+            try:
+                q.put(item, timeout=0.05)
+                return
+            except queue.Full:
+                if self.stop.is_set():
+                    raise _Stop() from None
+
+    def _get(self, q: queue.Queue):
+        while True:
+            try:
+                return q.get(timeout=0.05)
+            except queue.Empty:
+                if self.stop.is_set():
+                    raise _Stop() from None
+
+    def _reader(self) -> None:
+        from concurrent.futures import ThreadPoolExecutor
+
+        try:
+            with ThreadPoolExecutor(self.reader_workers, thread_name_prefix="product-read") as pool:
+                window = []  # reads in flight, in patch order
+                for patch in self.pending:
+                    # a decoded-tile slot, released when the patch is computed; while
+                    # none is free, hand the oldest read over so one can be
+                    while not self.slots.acquire(timeout=0.05):
+                        if self.stop.is_set():
+                            raise _Stop()
+                        if window:
+                            self._put(self.decoded, window.pop(0).result())
+                    window.append(pool.submit(self.read, patch))
+                    while window and (window[0].done() or len(window) >= self.reader_workers):
+                        self._put(self.decoded, window.pop(0).result())
+                while window:
+                    self._put(self.decoded, window.pop(0).result())
+        except _Stop:
+            pass
+        except BaseException as error:  # noqa: BLE001 - the first error ends the run
+            self.fail(error)
+        finally:
+            self.decoded.put(_END)
+
+    def _writer(self) -> None:
+        try:
+            while True:
+                item = self._get(self.computed)
+                if item is _END:
+                    break
+                self.write(*item)
+            self.finish()
+        except _Stop:
+            pass
+        except BaseException as error:  # noqa: BLE001
+            self.fail(error)
+
+    def run(self) -> None:
+        reader = threading.Thread(target=self._reader, name="product-reader")
+        writer = threading.Thread(target=self._writer, name="product-writer")
+        reader.start()
+        writer.start()
+        try:
+            while True:
+                item = self._get(self.decoded)
+                if item is _END:
+                    break
+                patch, tile = item
+                try:
+                    result = self.compute(patch, tile)
+                finally:
+                    self.slots.release()
+                self._put(self.computed, (patch, result))
+            self._put(self.computed, _END)
+        except _Stop:
+            pass
+        except BaseException as error:  # noqa: BLE001
+            self.fail(error)
+        finally:
+            reader.join()
+            writer.join()
+        if self.errors:
+            raise self.errors[0]
+
+
+_END = object()
+
+
 def _run(operation, dem_path, horizons_path, output_path, *, sun_vectors_m, times, backend,
          observer_height_m, invalid_value, compress, overwrite, verbose, progress_callback,
          progress_event_callback, cancellation_requested, device_id, bands_of, per_patch,
-         start_fresh=False):
+         start_fresh=False, output_transform=None, output_dtype=None, output_transform_id=None,
+         pipeline_mode="bounded", reader_worker_count=4, decoded_horizon_capacity=5,
+         writer_queue_capacity=1, durable_batch_size=16):
     _validate_common(backend, {"progress_callback": progress_callback,
                                "progress_event_callback": progress_event_callback,
                                "cancellation_requested": cancellation_requested}, invalid_value)
+    dtype = _validate_output_conversion(output_transform, output_dtype, output_transform_id)
+    _validate_pipeline(pipeline_mode, reader_worker_count, decoded_horizon_capacity,
+                       writer_queue_capacity, durable_batch_size)
     horizons, output = _preflight(horizons_path, output_path, overwrite)
     dem = _dem(dem_path)
     vectors = _vectors(sun_vectors_m, times)
@@ -151,40 +335,86 @@ def _run(operation, dem_path, horizons_path, output_path, *, sun_vectors_m, time
                                           code=f"{operation}_cancelled",
                                           details={"path": str(output)})
 
-    def blocks(tile, patch):
+    def read(patch):
+        """(patch, degrees [pixels, N] or None when the tile is missing or unreadable)."""
+        progress("read", progress.completed, len(patches), patch.tile_y, patch.tile_x)
+        try:
+            tile = store.read(patch.tile_y, patch.tile_x, observer_height_m)
+        except (OSError, ValueError):  # psr_pipeline.py:384-389: an unreadable tile is invalid
+            tile = None
+        if tile is not None:
+            # a ray that met no terrain (-inf in a .bin tile) sees the whole sky
+            tile = np.where(np.isfinite(tile), tile, np.float32(-90.0))
+        return patch, tile
+
+    def compute(patch, tile):
+        check_cancel()
+        if tile is None:
+            return None
+        progress("calculate", progress.completed, len(patches), patch.tile_y, patch.tile_x)
+        blocks = []
         for _, block in per_patch(calculator, dem, tile, prepared, patch):
             check_cancel()
-            yield block
+            blocks.append(np.asarray(block))
+        return blocks
 
     try:
         import hashlib
 
+        configuration = {"vectors_sha256": hashlib.sha256(vectors.tobytes()).hexdigest(),
+                         "observer_height_m": float(observer_height_m)}
+        if output_transform is not None:
+            configuration["output_transform_id"] = output_transform_id
         product = ResumableTiledProduct(
-            output, width=dem.width, height=dem.height, bands=bands, dtype=np.uint8,
+            output, width=dem.width, height=dem.height, bands=bands, dtype=dtype,
             affine_transform=dem.geo_transform, projection=dem.projection, compress=compress,
             descriptions=([str(t) for t in times]
                           if times is not None and len(times) == bands else None),
-            algorithm=operation,
-            configuration={"vectors_sha256": hashlib.sha256(vectors.tobytes()).hexdigest(),
-                           "observer_height_m": float(observer_height_m)},
+            algorithm=operation, configuration=configuration,
             timestamps_utc=(times if times is not None and len(times) == bands else ()),
             invalid_value=int(invalid_value), horizon_inventory_identity=inventory,
-            backend=selected, overwrite=overwrite, start_fresh=start_fresh)
-        progress("start", len(product.completed), len(patches))
-        for k, patch in enumerate(patches):
-            if product.is_complete(patch.tile_y, patch.tile_x):
-                continue
-            check_cancel()
-            tile = store.read(patch.tile_y, patch.tile_x, observer_height_m)
-            if tile is None:
-                product.write_invalid_patch(patch.tile_y, patch.tile_x, patch.height,
-                                            patch.width, int(invalid_value))
-                progress("invalid", k + 1, len(patches), patch.tile_y, patch.tile_x)
-                continue
-            # a ray that met no terrain (-inf in a .bin tile) sees the whole sky
-            tile = np.where(np.isfinite(tile), tile, np.float32(-90.0))
-            product.write_patch(patch.tile_y, patch.tile_x, blocks(tile, patch))
-            progress("valid", k + 1, len(patches), patch.tile_y, patch.tile_x)
+            backend=selected, overwrite=overwrite, start_fresh=start_fresh,
+            output_transform=output_transform)
+        progress.completed = len(product.completed)
+        progress("start", progress.completed, len(patches))
+        pending = [p for p in patches if not product.is_complete(p.tile_y, p.tile_x)]
+        by_key = {product.key(p.tile_y, p.tile_x): p for p in pending}
+        encode_workers = min(8, os.cpu_count() or 1) if bands > 1 else 0
+        with product.batch_writer(int(durable_batch_size), encode_workers=encode_workers) as batch:
+
+            def durable(keys):
+                for key in keys:
+                    patch = by_key[key]
+                    progress.completed += 1
+                    progress(product.completed[key], progress.completed, len(patches),
+                             patch.tile_y, patch.tile_x)
+
+            def write(patch, blocks):
+                progress("write", progress.completed, len(patches), patch.tile_y, patch.tile_x)
+                if blocks is None:
+                    durable(batch.write_invalid_patch(patch.tile_y, patch.tile_x, patch.height,
+                                                      patch.width, int(invalid_value)))
+                else:
+                    durable(batch.write_patch(patch.tile_y, patch.tile_x, blocks))
+
+            def finish():
+                durable(batch.checkpoint())
+
+            try:
+                if pipeline_mode == "serial":
+                    for patch in pending:
+                        check_cancel()
+                        _, tile = read(patch)
+                        write(patch, compute(patch, tile))
+                    finish()
+                else:
+                    _Overlap(pending, read, compute, write, finish,
+                             reader_workers=int(reader_worker_count),
+                             decoded_capacity=int(decoded_horizon_capacity),
+                             writer_capacity=int(writer_queue_capacity)).run()
+            finally:
+                durable(batch.checkpoint())  # whole patches written before an error stay
+        check_cancel()
         path = product.finalize()
         product = None
         progress("complete", len(patches), len(patches))
@@ -215,7 +445,12 @@ def generate_lightmap(dem_path, horizons_path, output_path, *, sun_vectors_m, ti
                       verbose: bool = False, progress_callback: Callable | None = None,
                       progress_event_callback: Callable | None = None,
                       cancellation_requested: Callable | None = None, device_id: int = 0,
-                      start_fresh: bool = False) -> Path:
+                      start_fresh: bool = False,
+                      output_transform: Callable | None = None, output_dtype=None,
+                      output_transform_id: str | None = None,
+                      pipeline_mode: str = "bounded", reader_worker_count: int = 4,
+                      decoded_horizon_capacity: int = 5, writer_queue_capacity: int = 1,
+                      durable_batch_size: int = 16) -> Path:
     """Tiled uint8 visible-solar-fraction GeoTIFF, one band per time (products.py:315-518)."""
 
     def bands_of(dem, vectors):
@@ -233,7 +468,13 @@ def generate_lightmap(dem_path, horizons_path, output_path, *, sun_vectors_m, ti
                 verbose=verbose, progress_callback=progress_callback,
                 progress_event_callback=progress_event_callback,
                 cancellation_requested=cancellation_requested, device_id=device_id,
-                bands_of=bands_of, per_patch=per_patch, start_fresh=start_fresh)
+                bands_of=bands_of, per_patch=per_patch, start_fresh=start_fresh,
+                output_transform=output_transform, output_dtype=output_dtype,
+                output_transform_id=output_transform_id, pipeline_mode=pipeline_mode,
+                reader_worker_count=reader_worker_count,
+                decoded_horizon_capacity=decoded_horizon_capacity,
+                writer_queue_capacity=writer_queue_capacity,
+                durable_batch_size=durable_batch_size)
 
 
 def generate_psr(dem_path, horizons_path, output_path, *, sun_vectors_m, times=None,
@@ -242,7 +483,12 @@ def generate_psr(dem_path, horizons_path, output_path, *, sun_vectors_m, times=N
                  progress_callback: Callable | None = None,
                  progress_event_callback: Callable | None = None,
                  cancellation_requested: Callable | None = None, device_id: int = 0,
-                 start_fresh: bool = False) -> Path:
+                 start_fresh: bool = False,
+                 output_transform: Callable | None = None, output_dtype=None,
+                 output_transform_id: str | None = None,
+                 pipeline_mode: str = "bounded", reader_worker_count: int = 4,
+                 decoded_horizon_capacity: int = 5, writer_queue_capacity: int = 1,
+                 durable_batch_size: int = 16) -> Path:
     """Single-band uint8 PSR classification GeoTIFF, 255 = PSR (products.py:520-698).
 
     The Sun vectors are first reduced to the per-azimuth-bin maxima seen from
@@ -265,4 +511,10 @@ def generate_psr(dem_path, horizons_path, output_path, *, sun_vectors_m, times=N
                 verbose=verbose, progress_callback=progress_callback,
                 progress_event_callback=progress_event_callback,
                 cancellation_requested=cancellation_requested, device_id=device_id,
-                bands_of=bands_of, per_patch=per_patch, start_fresh=start_fresh)
+                bands_of=bands_of, per_patch=per_patch, start_fresh=start_fresh,
+                output_transform=output_transform, output_dtype=output_dtype,
+                output_transform_id=output_transform_id, pipeline_mode=pipeline_mode,
+                reader_worker_count=reader_worker_count,
+                decoded_horizon_capacity=decoded_horizon_capacity,
+                writer_queue_capacity=writer_queue_capacity,
+                durable_batch_size=durable_batch_size)

@@ -172,3 +172,27 @@ def test_device_inputs_are_checked_before_any_kernel_reads_them():
         _check_device_vectors(_fake_device_buffer((39, 3), np.float64), "sun", 40)
     with pytest.raises(InputError, match="float64"):
         _check_device_vectors(_fake_device_buffer((40, 3), np.float32), "earth")
+
+
+def test_host_lighting_reads_either_layout_to_the_same_bits():
+    """The host build walks [pixel][azimuth] and [azimuth][pixel] rows alike."""
+    dem = synthetic.fbm_crater_dem(256, 20.0, 12)
+    rng = np.random.default_rng(12)
+    az = np.arange(1440) * (2 * np.pi / 1440)
+    # smooth rows with steep walls, so the disk walk and the lit/dark exits both run
+    base = rng.normal(0.0, 2.0, (2, 16384, 1)) + 3.0 * np.sin(az[None, None, :] * rng.integers(
+        1, 5, (2, 16384, 1)) + rng.uniform(0, 6.3, (2, 16384, 1)))
+    horizons = np.maximum(base, -50.0).astype(np.float32)
+    tiles = [(0, 0), (128, 128)]
+    sun = synthetic.sun_vectors(dem, 48, 2.0, seed=13)
+    earth = synthetic.earth_vectors(48, 2.0, seed=14)
+    cpu = CpuLightingSession(threads=4)
+    names = ("sun_bytes", "sun_fraction", "sun_mean", "earth_margin", "earth_visible", "psr",
+             "psr_margin")
+    got = [cpu.lighting_tiles(dem, tiles, data, sun=sun, earth=earth, psr_vectors=sun[:24],
+                              azimuth_major=major, **{k: True for k in names})
+           for major, data in ((False, horizons),
+                               (True, np.ascontiguousarray(horizons.transpose(0, 2, 1))))]
+    for k in names:
+        np.testing.assert_array_equal(got[0][k], got[1][k], err_msg=k)
+    assert 0.0 < float(got[0]["sun_fraction"].mean()) < 1.0
