@@ -80,7 +80,13 @@ def parse_args():
     p.add_argument("--batch-tiles", type=int, default=48,
                    help="tiles of the 48-site workload timed plain and batched (0 skips)")
     p.add_argument("--pipeline-tiles", type=int, default=12,
-                   help="tiles of the end-to-end generate_horizons leg (0 skips)")
+                   help="tiles (a block of the site's top-left corner, 4 across) of the "
+                        "end-to-end generate_horizons and product legs (0 skips)")
+    p.add_argument("--no-products", dest="products", action="store_false",
+                   help="skip the lightmap / PSR products over the generated tiles")
+    p.add_argument("--pixel-major", action="store_true",
+                   help="keep resident horizons [pixel][azimuth] (the tile file layout) "
+                        "instead of [azimuth][pixel]")
     p.add_argument("--no-year-lighting", dest="year_lighting", action="store_false",
                    help="skip the 365-day lighting pass over resident horizons")
     p.add_argument("--no-adjoint", dest="adjoint", action="store_false",
@@ -120,7 +126,9 @@ def main() -> int:
     grid = [(x, y) for y in range(0, primary.height, 128) for x in range(0, primary.width, 128)]
     tiles = [grid[(i * 97) % len(grid)] for i in range(len(grid))]
     per_step = args.tiles_per_step
-    horizons = DeviceBuffer.empty(session, (per_step, PIXELS, AZIMUTHS), np.float32)
+    major = not args.pixel_major
+    horizons = DeviceBuffer.empty(session, (per_step, AZIMUTHS, PIXELS) if major
+                                  else (per_step, PIXELS, AZIMUTHS), np.float32)
     segments = None
     outputs = None
     stats = {"march_ms": 0.0, "march_launches": 0, "segments_ms": 0.0, "lighting_ms": 0.0,
@@ -135,13 +143,15 @@ def main() -> int:
             segments = session.segments_device(dems, tile_column=c, tile_row=r, out=segments)
             e1 = Event(session).record()
             session.march_device(dems, segments, tile_column=c, tile_row=r,
-                                 flags=HORIZON_DEGREES, output=horizons.slot(slot))
+                                 flags=HORIZON_DEGREES, output=horizons.slot(slot),
+                                 azimuth_major=major)
             e2 = Event(session).record()
             events.append((e0, e1, e2))
         l0 = Event(session).record()
         outputs = lighting.lighting_tiles(primary, batch, horizons, sun=sun_d, earth=earth_d,
                                           psr_vectors=psr_d, sun_bytes=True, sun_mean=True,
-                                          earth_visible=True, psr=True, outputs=outputs)
+                                          earth_visible=True, psr=True, outputs=outputs,
+                                          azimuth_major=major)
         l1 = Event(session).record()
         if record:
             for e0, e1, e2 in events:
@@ -190,12 +200,15 @@ def main() -> int:
         counters.free()
         counted = {"tiles": len(pick), "samples": samples / len(pick), "tests": tests / len(pick)}
     roofline = _march_roofline(counted, march_ms, len(dems))
-    codec = _codec_leg(horizons)
+    # the tile file layout [pixel][azimuth] of one resident tile, for the codec and CPU legs
     cpu_light_tile = horizons.slot(0).to_host()
+    if major:
+        cpu_light_tile = np.ascontiguousarray(cpu_light_tile.T)
+    codec = _codec_leg(cpu_light_tile)
     long_lighting = reducers = None
     if args.year_lighting:
-        long_lighting = _year_lighting(session, lighting, primary, horizons, resident)
-        reducers = _durations_leg(session, lighting, primary, horizons, resident)
+        long_lighting = _year_lighting(session, lighting, primary, horizons, resident, major)
+        reducers = _durations_leg(session, lighting, primary, horizons, resident, major)
     horizons.free()
     if segments is not None:
         segments.free()
@@ -249,7 +262,12 @@ def main() -> int:
     if args.batch_tiles > 0:
         result["batched"] = _batched_vs_plain(session, dems[1], args.batch_tiles)
     if args.pipeline_tiles > 0:
-        result["generate_horizons"] = _pipeline_leg(session, dems, tiles[:args.pipeline_tiles])
+        corner = [(c * 128, r * 128) for r in range(0, (args.pipeline_tiles + 3) // 4)
+                  for c in range(4)][:args.pipeline_tiles]
+        with tempfile.TemporaryDirectory(prefix="lsh-bench-") as root:
+            result["generate_horizons"] = _pipeline_leg(session, dems, corner, Path(root))
+            if args.products:
+                result["products"] = _products_leg(primary, corner, Path(root))
     if args.adjoint:
         result["adjoint"] = _adjoint_leg(session)
     if args.cpu_sample_pixels > 0:
@@ -293,11 +311,10 @@ def _march_roofline(counted, march_ms: float, passes: int) -> dict:
     return out
 
 
-def _codec_leg(horizons):
+def _codec_leg(degrees):
     """.cbin encode / decode of one marched full-site tile on the host threads."""
     from lunarscout_amd import tiles
 
-    degrees = horizons.slot(0).to_host()
     payload = tiles.encode_cbin(degrees)
     t0 = time.perf_counter()
     for _ in range(3):
@@ -313,7 +330,7 @@ def _codec_leg(horizons):
             "threads": len(os.sched_getaffinity(0))}
 
 
-def _year_lighting(session, lighting, primary, horizons, batch):
+def _year_lighting(session, lighting, primary, horizons, batch, major):
     """Long-duration lighting from stored horizons: 4380 Sun and 4380 Earth
     vectors (2-hour steps over 365 days, seeds 5 and 6) over the resident tiles,
     per-time sun-fraction bytes + Earth visibility + PSR in one streamed launch."""
@@ -326,7 +343,7 @@ def _year_lighting(session, lighting, primary, horizons, batch):
     psr_vectors, _ = reduce_sun_vectors_for_psr(primary, sun)
     bufs = [DeviceBuffer.from_host(session, v) for v in (sun, earth, psr_vectors)]
     kwargs = dict(sun=bufs[0], earth=bufs[1], psr_vectors=bufs[2], sun_bytes=True, sun_mean=True,
-                  earth_visible=True, psr=True)
+                  earth_visible=True, psr=True, azimuth_major=major)
     outputs = lighting.lighting_tiles(primary, batch, horizons, **kwargs)  # warm-up
     e0 = Event(session).record()
     outputs = lighting.lighting_tiles(primary, batch, horizons, outputs=outputs, **kwargs)
@@ -341,6 +358,7 @@ def _year_lighting(session, lighting, primary, horizons, batch):
     achieved = flops / (ms / 1000.0) / 1e12
     return {"workload": f"{len(batch)} resident full-site tiles x (4380 Sun + 4380 Earth + "
                         f"{len(psr_vectors)} PSR vectors), 2-hour steps over 365 days",
+            "layout": "[azimuth][pixel]" if major else "[pixel][azimuth]",
             "value": round(px_times / (ms / 1000.0), 1), "unit": "px·times/s",
             "kernel_ms": round(ms, 3),
             "roofline": {"kernel": "lighting_kernel", "bound": "valu", "unit": "TFLOP/s",
@@ -353,7 +371,7 @@ def _year_lighting(session, lighting, primary, horizons, batch):
                          .get("valu_busy")}}
 
 
-def _durations_leg(session, lighting, primary, horizons, batch):
+def _durations_leg(session, lighting, primary, horizons, batch, major):
     """Downstream reducers fused into the lighting walk over the resident tiles:
     the longest sunlight (fraction >= 0.5) AND Earth-visible (margin >= 0 deg)
     duration per calendar-month start interval, and the monthly safe haven (Sun
@@ -381,9 +399,11 @@ def _durations_leg(session, lighting, primary, horizons, batch):
     result = {"workload": f"{len(batch)} resident full-site tiles x 4380 times (Sun + Earth), "
                           "2-hour steps over 365 days; 12 monthly bands each"}
     for name, plan in plans.items():
-        out = lighting.duration_tiles(primary, batch, horizons, plan, bufs[0], bufs[1])  # warm-up
+        out = lighting.duration_tiles(primary, batch, horizons, plan, bufs[0], bufs[1],
+                                      azimuth_major=major)  # warm-up
         e0 = Event(session).record()
-        out = lighting.duration_tiles(primary, batch, horizons, plan, bufs[0], bufs[1], out=out)
+        out = lighting.duration_tiles(primary, batch, horizons, plan, bufs[0], bufs[1], out=out,
+                                      azimuth_major=major)
         e1 = Event(session).record()
         ms = e0.elapsed_ms(e1)
         values = out.to_host()
@@ -527,11 +547,11 @@ def _batched_vs_plain(session, regional, tile_count):
     }
 
 
-def _pipeline_leg(session, dems, tiles):
-    """generate_horizons' path end to end on a subset of the full site: the
+def _pipeline_leg(session, dems, tiles, root):
+    """generate_horizons' path end to end on a block of the full site: the
     bounded pipeline (GPU segments + march, degrees copied back, .cbin encode
-    and the staged, fsynced tile publish in the writer thread) into a fresh
-    directory; tiles/s from the wall clock."""
+    and the staged, fsynced tile publish in the writer thread) into the empty
+    directory ``root``; tiles/s from the wall clock."""
     from lunarscout_amd.pipeline import PatchDescriptor, run_bounded_pipeline
     from lunarscout_amd.tiles import HorizonTileStore
 
@@ -543,19 +563,51 @@ def _pipeline_leg(session, dems, tiles):
                                         degrees=True)
         return process
 
-    with tempfile.TemporaryDirectory(prefix="lsh-bench-") as root:
-        started = time.perf_counter()
-        result = run_bounded_pipeline(patches, store=HorizonTileStore(root),
-                                      prepare_patch=lambda _p: None, processor_factory=factory,
-                                      compress=True, skip_existing=False,
-                                      prepared_queue_capacity=1, writer_queue_capacity=1)
-        wall = time.perf_counter() - started
-        size = sum(p.stat().st_size for p in Path(root).rglob("*.cbin"))
-    return {"workload": f"{len(tiles)} full-site tiles through generate_horizons' bounded "
-                        "pipeline, compressed (.cbin) staged writes",
+    started = time.perf_counter()
+    result = run_bounded_pipeline(patches, store=HorizonTileStore(root),
+                                  prepare_patch=lambda _p: None, processor_factory=factory,
+                                  compress=True, skip_existing=False,
+                                  prepared_queue_capacity=1, writer_queue_capacity=1)
+    wall = time.perf_counter() - started
+    size = sum(p.stat().st_size for p in Path(root).rglob("*.cbin"))
+    return {"workload": f"{len(tiles)} full-site tiles (the top-left block) through "
+                        "generate_horizons' bounded pipeline, compressed (.cbin) staged writes",
             "tiles_per_s": round(len(tiles) / wall, 4), "wall_s": round(wall, 3),
             "compute_s": round(result.compute_seconds, 3), "write_s": round(result.write_seconds, 3),
             "bytes_written": int(size)}
+
+
+def _products_leg(primary, tiles, root):
+    """generate_psr and generate_lightmap end to end (hip backend) over the
+    tiles the generate_horizons leg wrote: the DEM block those tiles cover, tile
+    reads + .cbin decodes on 4 threads, the GPU calculation, compressed GeoTIFF
+    blocks and batched durable checkpoints, overlapped by the bounded pipeline.
+    PSR from a year of 2-hourly Sun vectors (4380, reduced); the lightmap from
+    two years of 6-hourly vectors (2921 bands), the reference's two published
+    product workloads; patches/s from the wall clock."""
+    from lunarscout_amd import synthetic
+    from lunarscout_amd.geometry import DemGrid
+    from lunarscout_amd.products import generate_lightmap, generate_psr
+
+    height = max(r for _, r in tiles) + 128
+    width = max(c for c, _ in tiles) + 128
+    dem = DemGrid(np.ascontiguousarray(primary.elevation_m[:height, :width]),
+                  primary.geo_transform, primary.projection)
+    out = {"workload": f"{len(tiles)} patches of the full site's {width}x{height} px top-left "
+                       "block, from the generate_horizons leg's .cbin tiles"}
+    for name, fn, count, hours in (("psr", generate_psr, 4380, 2.0),
+                                   ("lightmap", generate_lightmap, 2921, 6.0)):
+        vectors = synthetic.sun_vectors(primary, count, hours, seed=5)
+        path = root / f"{name}.tif"
+        started = time.perf_counter()
+        fn(dem, root, path, sun_vectors_m=vectors, backend="hip", overwrite=True)
+        wall = time.perf_counter() - started
+        out[name] = {"vectors": count, "bands": 1 if name == "psr" else count,
+                     "patches_per_s": round(len(tiles) / wall, 4), "wall_s": round(wall, 3),
+                     "bytes_written": path.stat().st_size}
+        path.unlink()
+        log(f"{name} product: {len(tiles)} patches in {wall:.2f} s")
+    return out
 
 
 def _cpu_baseline(dems, sample_pixels, horizon_tile, tile):
