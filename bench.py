@@ -305,9 +305,18 @@ def _march_roofline(counted, march_ms: float, passes: int) -> dict:
         # the four 4-byte level-0 taps per sample, the segment corners and the output
         gathered = 4.0 * counted["tests"] + 16.0 * counted["samples"] + 256.0 * rays * passes \
             + 4.0 * rays
+        # the method's 2x2 gather: 16 B per bound test and per sample (the kernel
+        # reads a test's bound as one stored 4 B value instead)
+        algorithmic = 16.0 * (counted["tests"] + counted["samples"]) + 256.0 * rays * passes \
+            + 4.0 * rays
+        seconds = march_ms / 1000.0
         out.update({"achieved": round(achieved, 3), "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4),
                     "flop_per_launch": flops, "gathered_bytes_per_launch": gathered,
-                    "hbm_frac_of_gathered": round(gathered / (march_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)})
+                    "gathered_gbs": round(gathered / seconds / 1e9, 1),
+                    "algorithmic_bytes_per_launch": algorithmic,
+                    "algorithmic_gbs": round(algorithmic / seconds / 1e9, 1)})
+        if out["traffic"]:
+            out["traffic_frac_of_hbm_peak"] = round(out["traffic"] / seconds / 1e9 / HBM_PEAK_GBS, 5)
     return out
 
 
