@@ -33,12 +33,14 @@ namespace {
 constexpr int kBlockPixels = 8;       // 8x8 pixel block per wavefront
 constexpr int kAzimuthsPerGroup = 4;  // wavefronts per workgroup
 
-// The march is latency-bound (dependent gathers, FP64 chains): on the full-site
-// tiles, capping registers for 5 wavefronts per SIMD (96 VGPRs, a few spills
-// outside the hottest path) measured 405 ms per tile against 418 ms at 4 and
-// 448 ms at 6 wavefronts; the uncapped build holds 2.
+// Registers are sized for 4 wavefronts per SIMD (128 VGPRs). The level test
+// forms the cell exit while its bound load is in flight, which keeps more
+// values live; at 4 wavefronts the plain kernel spills 44 B per lane and the
+// batched one 100 B. On the full-site tiles this measured 367 ms per tile
+// against 379 ms at 5 wavefronts (96 VGPRs, 152 B of spills; the batched
+// kernel 1.6x slower there); the uncapped build holds 2.
 #ifndef LSH_MARCH_WAVES_PER_EU
-#define LSH_MARCH_WAVES_PER_EU 5
+#define LSH_MARCH_WAVES_PER_EU 4
 #endif
 
 // One ray fan of one pixel at one azimuth over every DEM pass, written to

@@ -68,9 +68,6 @@ struct PassSegment {
   S sx, sy;
   S a1, a2, a3, a4, b1, b2, b3, b4;
   S s_start, s_end, s_chord, c1, c2, c3;
-  // tangent coefficients 2 a2, 3 a3, 4 a4 (and b), the products tangent()
-  // forms first (cuda_backend.py:73-84)
-  S ta2, ta3, ta4, tb2, tb3, tb4;
 };
 
 struct MarchCounters {
@@ -103,23 +100,21 @@ LSH_HD S lsh_fma(S a, S b, S c) {
                         : static_cast<S>(__builtin_fma(double(a), double(b), double(c)));
 }
 
-// evaluate (cuda_backend.py:62-71) on the start offset, in Horner form with
-// fused multiply-adds: x0 + d (a1 + d (a2 + d (a3 + d a4))).
+// evaluate and tangent (cuda_backend.py:62-84) at the start offset d, in one
+// Horner pass with fused multiply-adds: the partial sums b3 = a4 d + a3,
+// b2 = b3 d + a2, b1 = b2 d + a1 give the position b1 d + x0, and the same
+// recurrence on them gives the derivative 4 a4 d^3 + 3 a3 d^2 + 2 a2 d + a1
+// (c3 = a4 d + b3, c2 = c3 d + b2, c2 d + b1), with no per-ray 2 a2, 3 a3, 4 a4.
 template <class S>
-LSH_HD S lsh_quartic(S x0, S a1, S a2, S a3, S a4, S d) {
-  S r = lsh_fma(d, a4, a3);
-  r = lsh_fma(d, r, a2);
-  r = lsh_fma(d, r, a1);
-  return lsh_fma(d, r, x0);
-}
-
-// tangent (cuda_backend.py:73-84) in Horner form: a1 + d (2 a2 + d (3 a3 + d 4 a4)),
-// with t2 = 2 a2, t3 = 3 a3, t4 = 4 a4 formed once per ray.
-template <class S>
-LSH_HD S lsh_tangent(S a1, S t2, S t3, S t4, S d) {
-  S r = lsh_fma(d, t4, t3);
-  r = lsh_fma(d, r, t2);
-  return lsh_fma(d, r, a1);
+LSH_HD void lsh_quartic_and_tangent(S x0, S a1, S a2, S a3, S a4, S d, S& position,
+                                    S& tangent) {
+  const S b3 = lsh_fma(d, a4, a3);
+  const S b2 = lsh_fma(d, b3, a2);
+  const S b1 = lsh_fma(d, b2, a1);
+  position = lsh_fma(d, b1, x0);
+  const S c3 = lsh_fma(d, a4, b3);
+  const S c2 = lsh_fma(d, c3, b2);
+  tangent = lsh_fma(d, c2, b1);
 }
 
 template <class T>
@@ -299,8 +294,9 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   while (s.at_most(g.s_end)) {
     // -- position (cuda_backend.py:672-705)
     const S delta = s.minus(g.s_start);
-    const S px = lsh_quartic(g.sx, g.a1, g.a2, g.a3, g.a4, delta);
-    const S py = lsh_quartic(g.sy, g.b1, g.b2, g.b3, g.b4, delta);
+    S px, py, dx, dy;
+    lsh_quartic_and_tangent(g.sx, g.a1, g.a2, g.a3, g.a4, delta, px, dx);
+    lsh_quartic_and_tangent(g.sy, g.b1, g.b2, g.b3, g.b4, delta, py, dy);
     // the reference's px in [0, W-1), py in [0, H-1), tested on the absolute cell
     // index floor(px) + bx (exact: integers); NaN fails every comparison
     const S fx = lsh_floor(px), fy = lsh_floor(py);
@@ -324,49 +320,64 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     int level = (lsh_exponent(footprint) - d.resolution_exponent + 1) >> 1;
     if (!(lsh_ldexp(res, 2 * level) <= footprint)) --level;
     level = footprint >= res ? (level < top_level ? level : top_level) : 0;
-    const S dx = lsh_tangent(g.a1, g.ta2, g.ta3, g.ta4, delta);
-    const S dy = lsh_tangent(g.b1, g.tb2, g.tb3, g.tb4, delta);
+
     // -- hierarchy tests at this position (cuda_backend.py:706-868)
-    // Each level's outcome -- the ray leaves the level (it ends), a nodata
-    // block, a cull, a level-0 sample, or a descent -- is formed without
-    // branches and the loop has one exit.
-    bool ray_out, nodata, culled;
-    int cx, cy, scale;
-    float maxh;
-    while (true) {
-      const int shift = level * 2;
-      scale = 1 << shift;
-      cx = ix >> shift;
-      cy = iy >> shift;
-      const int w = d.level_width[level], h = d.level_height[level];
-      ray_out = !(cx < w && cy < h);
-      // hierarchy_bilinear_bound (cuda_backend.py:47-60) -- the maximum of the
-      // valid elevations of cells (cx..cx+1, cy..cy+1) clipped to the level,
-      // -32000 when none is valid -- read from the bound copy of the level,
-      // where it is stored per cell (one load instead of four)
-      const uint32_t at = ray_out ? 0u : static_cast<uint32_t>(d.level_offset[level] + cy * w + cx);
-      maxh = lsh_load(d, static_cast<uint32_t>(d.bound_offset) + at);
-      nodata = maxh < -20000.0f;
-      const S side = lsh_ldexp(res, 2 * level);  // 4^level res
-      const V true_near = static_cast<V>(lsh_max(true_m - side, S(1)));
-      culled = lsh_bound_cannot_rise(static_cast<V>(maxh) - oz, true_near, near, two_radius_oz,
-                                     inv_two_observer_radius, current);
-      if (kCount && !ray_out) {
-        ++cnt.steps;
-        if (nodata || culled || level != 0) ++cnt.tests;
-      }
-      if (ray_out || nodata || culled || level == 0) break;
-      trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx),
-                float(py + g.by), maxh, 0.f, 0.f, 0.f, 0);
-      --level;
+    // hierarchy_bilinear_bound (cuda_backend.py:47-60) -- the maximum of the
+    // valid elevations of cells (cx..cx+1, cy..cy+1) clipped to the level,
+    // -32000 when none is valid -- is read from the bound copy of the level,
+    // where it is stored per cell (one load instead of four). The start
+    // level's bound is loaded first, and the cell exit and fallback of that
+    // level, which do not depend on it, are formed while the load is in flight.
+    // A descent (the bound neither culls nor is at level 0, rare) tests the
+    // finer levels in turn and forms the exit of the level it stops at. The
+    // reference also ends a ray whose cell lies outside the level; the level
+    // sizes are ceil(W / 4^l) x ceil(H / 4^l), so the cell of a position in
+    // [0, W-2] x [0, H-2] lies inside every level and that test never fires.
+    int shift = level * 2;
+    int scale = 1 << shift;
+    int cx = ix >> shift, cy = iy >> shift;
+    float maxh = lsh_load(d, static_cast<uint32_t>(d.bound_offset + d.level_offset[level] +
+                                                   cy * d.level_width[level] + cx));
+    S fallback = (lsh_ldexp(res, shift) * S(0.5)) * S(0.001);
+    S exit_distance = lsh_cell_exit(static_cast<S>(cx * scale - g.bx) - px,
+                                    static_cast<S>(cy * scale - g.by) - py,
+                                    static_cast<S>(scale), dx, dy);
+    bool nodata = maxh < -20000.0f;
+    bool culled = lsh_bound_cannot_rise(
+        static_cast<V>(maxh) - oz, static_cast<V>(lsh_max(true_m - lsh_ldexp(res, shift), S(1))),
+        near, two_radius_oz, inv_two_observer_radius, current);
+    if (kCount) {
+      ++cnt.steps;
+      if (nodata || culled || level != 0) ++cnt.tests;
     }
-    if (ray_out) break;
+    if (!(nodata || culled || level == 0)) {
+      do {
+        trace.row(float(s.value()), float(true_m), level, cx, cy, float(px + g.bx),
+                  float(py + g.by), maxh, 0.f, 0.f, 0.f, 0);
+        --level;
+        shift = level * 2;
+        scale = 1 << shift;
+        cx = ix >> shift;
+        cy = iy >> shift;
+        maxh = lsh_load(d, static_cast<uint32_t>(d.bound_offset + d.level_offset[level] +
+                                                 cy * d.level_width[level] + cx));
+        nodata = maxh < -20000.0f;
+        culled = lsh_bound_cannot_rise(
+            static_cast<V>(maxh) - oz,
+            static_cast<V>(lsh_max(true_m - lsh_ldexp(res, shift), S(1))), near, two_radius_oz,
+            inv_two_observer_radius, current);
+        if (kCount) {
+          ++cnt.steps;
+          if (nodata || culled || level != 0) ++cnt.tests;
+        }
+      } while (!(nodata || culled || level == 0));
+      fallback = (lsh_ldexp(res, shift) * S(0.5)) * S(0.001);
+      exit_distance = lsh_cell_exit(static_cast<S>(cx * scale - g.bx) - px,
+                                    static_cast<S>(cy * scale - g.by) - py,
+                                    static_cast<S>(scale), dx, dy);
+    }
     const bool sampled = !nodata && !culled;  // at level 0
     // -- cell skip (cull / nodata) and level-0 step, selected per lane
-    const S fscale = static_cast<S>(scale);
-    const S fallback = (lsh_ldexp(res, 2 * level) * S(0.5)) * S(0.001);
-    const S exit_distance = lsh_cell_exit(static_cast<S>(cx * scale - g.bx) - px,
-                                          static_cast<S>(cy * scale - g.by) - py, fscale, dx, dy);
     // the distance to the cell exit, or the fallback when the exit is not ahead
     // (cuda_backend.py:756-763): a cull or nodata skip moves 1e-4 km past it
     const S to_exit = exit_distance > S(0) ? exit_distance : fallback;
@@ -527,8 +538,6 @@ LSH_HD PassSegment<S> lsh_interpolate(const float* p00, const float* p10, const 
   g.s_start = LSH_FIELD(12); g.s_end = LSH_FIELD(13); g.s_chord = LSH_FIELD(14);
   g.c1 = LSH_FIELD(15); g.c2 = LSH_FIELD(16); g.c3 = LSH_FIELD(17);
 #undef LSH_FIELD
-  g.ta2 = S(2) * g.a2; g.ta3 = S(3) * g.a3; g.ta4 = S(4) * g.a4;
-  g.tb2 = S(2) * g.b2; g.tb3 = S(3) * g.b3; g.tb4 = S(4) * g.b4;
   return g;
 }
 
