@@ -327,7 +327,7 @@ def _float_product(operation: str, *, dem, horizon_store, output_path, band_coun
     patch not yet journaled (``start_fresh`` discards a staged product)."""
     from .lighting import select_lighting_session
     from .pipeline import enumerate_patches
-    from .products import _dem, _Progress
+    from .products import _dem, _Progress, drive_patches
     from .tiles import HorizonTileStore
 
     if backend not in ("auto", "cpu", "hip"):
@@ -365,9 +365,8 @@ def _float_product(operation: str, *, dem, horizon_store, output_path, band_coun
                                           code=f"{operation}_cancelled",
                                           details={"path": str(output)})
 
-    def bands(tile, patch):
+    def bands(patch, tile):
         for block in compute(calculator, dem, tile, patch):
-            check_cancel()
             block = np.asarray(block, dtype=np.float32)
             if not np.isnan(nodata):
                 block = np.where(np.isnan(block), np.float32(nodata), block)
@@ -382,20 +381,12 @@ def _float_product(operation: str, *, dem, horizon_store, output_path, band_coun
             timestamps_utc=timestamps_utc, invalid_value=nodata,
             horizon_inventory_identity=inventory, backend=selected,
             overwrite=overwrite, start_fresh=start_fresh)
-        progress("start", len(product.completed), len(patches))
-        for k, patch in enumerate(patches):
-            if product.is_complete(patch.tile_y, patch.tile_x):
-                continue
-            check_cancel()
-            tile = store.read(patch.tile_y, patch.tile_x, observer_elevation_m)
-            if tile is None:
-                product.write_invalid_patch(patch.tile_y, patch.tile_x, patch.height,
-                                            patch.width, nodata)
-                progress("invalid", k + 1, len(patches), patch.tile_y, patch.tile_x)
-                continue
-            tile = np.where(np.isfinite(tile), tile, np.float32(-90.0))
-            product.write_patch(patch.tile_y, patch.tile_x, bands(tile, patch))
-            progress("valid", k + 1, len(patches), patch.tile_y, patch.tile_x)
+        progress.completed = len(product.completed)
+        progress("start", progress.completed, len(patches))
+        # tile reads, the fused reductions and the writes overlap (drive_patches)
+        drive_patches(product, patches, store, observer_elevation_m, compute=bands,
+                      progress=progress, check_cancel=check_cancel, invalid_fill=nodata)
+        check_cancel()
         path = product.finalize()
         product = None
         progress("complete", len(patches), len(patches))

@@ -372,7 +372,7 @@ class ProductBatchWriter:
                             fill) -> list[str]:
         """Every band holds ``fill``: one block encoded once for all bands."""
         writer = self.product.writer
-        shape = (height, width, fill)
+        shape = (height, width, repr(fill))  # NaN fills compare equal by their repr
         payload = self.invalid_payloads.get(shape)
         if payload is None:
             payload = writer.encode_block(np.full((height, width), fill, dtype=writer.dtype))

@@ -299,6 +299,82 @@ class _Overlap:
 _END = object()
 
 
+def drive_patches(product, patches, store, observer_height_m, *, compute, progress, check_cancel,
+                  invalid_fill, pipeline_mode="bounded", reader_worker_count=4,
+                  decoded_horizon_capacity=5, writer_queue_capacity=1,
+                  durable_batch_size=16) -> None:
+    """Every patch of ``patches`` not yet complete in ``product``: read its
+    horizon tile, ``compute(patch, degrees)`` -> band blocks (None when the
+    tile is missing or unreadable: the patch is written with ``invalid_fill``),
+    write and checkpoint, serially or overlapped (``_Overlap``). ``progress``
+    receives read / calculate / write events and, when a patch is durable, its
+    state; ``progress.completed`` counts durable patches. Whole patches written
+    before an error are checkpointed before it propagates."""
+    pending = [p for p in patches if not product.is_complete(p.tile_y, p.tile_x)]
+    by_key = {product.key(p.tile_y, p.tile_x): p for p in pending}
+    total = len(patches)
+
+    def read(patch):
+        """(patch, degrees [pixels, N] or None when the tile is missing or unreadable)."""
+        progress("read", progress.completed, total, patch.tile_y, patch.tile_x)
+        try:
+            tile = store.read(patch.tile_y, patch.tile_x, observer_height_m)
+        except (OSError, ValueError):  # psr_pipeline.py:384-389: an unreadable tile is invalid
+            tile = None
+        if tile is not None:
+            # a ray that met no terrain (-inf in a .bin tile) sees the whole sky
+            tile = np.where(np.isfinite(tile), tile, np.float32(-90.0))
+        return patch, tile
+
+    def calculate(patch, tile):
+        check_cancel()
+        if tile is None:
+            return None
+        progress("calculate", progress.completed, total, patch.tile_y, patch.tile_x)
+        blocks = []
+        for block in compute(patch, tile):
+            check_cancel()
+            blocks.append(np.asarray(block))
+        return blocks
+
+    bands = product.writer.bands
+    encode_workers = min(8, os.cpu_count() or 1) if bands > 1 else 0
+    with product.batch_writer(int(durable_batch_size), encode_workers=encode_workers) as batch:
+
+        def durable(keys):
+            for key in keys:
+                patch = by_key[key]
+                progress.completed += 1
+                progress(product.completed[key], progress.completed, total, patch.tile_y,
+                         patch.tile_x)
+
+        def write(patch, blocks):
+            progress("write", progress.completed, total, patch.tile_y, patch.tile_x)
+            if blocks is None:
+                durable(batch.write_invalid_patch(patch.tile_y, patch.tile_x, patch.height,
+                                                  patch.width, invalid_fill))
+            else:
+                durable(batch.write_patch(patch.tile_y, patch.tile_x, blocks))
+
+        def finish():
+            durable(batch.checkpoint())
+
+        try:
+            if pipeline_mode == "serial":
+                for patch in pending:
+                    check_cancel()
+                    _, tile = read(patch)
+                    write(patch, calculate(patch, tile))
+                finish()
+            else:
+                _Overlap(pending, read, calculate, write, finish,
+                         reader_workers=int(reader_worker_count),
+                         decoded_capacity=int(decoded_horizon_capacity),
+                         writer_capacity=int(writer_queue_capacity)).run()
+        finally:
+            durable(batch.checkpoint())  # whole patches written before an error stay
+
+
 def _run(operation, dem_path, horizons_path, output_path, *, sun_vectors_m, times, backend,
          observer_height_m, invalid_value, compress, overwrite, verbose, progress_callback,
          progress_event_callback, cancellation_requested, device_id, bands_of, per_patch,
@@ -335,29 +411,6 @@ def _run(operation, dem_path, horizons_path, output_path, *, sun_vectors_m, time
                                           code=f"{operation}_cancelled",
                                           details={"path": str(output)})
 
-    def read(patch):
-        """(patch, degrees [pixels, N] or None when the tile is missing or unreadable)."""
-        progress("read", progress.completed, len(patches), patch.tile_y, patch.tile_x)
-        try:
-            tile = store.read(patch.tile_y, patch.tile_x, observer_height_m)
-        except (OSError, ValueError):  # psr_pipeline.py:384-389: an unreadable tile is invalid
-            tile = None
-        if tile is not None:
-            # a ray that met no terrain (-inf in a .bin tile) sees the whole sky
-            tile = np.where(np.isfinite(tile), tile, np.float32(-90.0))
-        return patch, tile
-
-    def compute(patch, tile):
-        check_cancel()
-        if tile is None:
-            return None
-        progress("calculate", progress.completed, len(patches), patch.tile_y, patch.tile_x)
-        blocks = []
-        for _, block in per_patch(calculator, dem, tile, prepared, patch):
-            check_cancel()
-            blocks.append(np.asarray(block))
-        return blocks
-
     try:
         import hashlib
 
@@ -377,43 +430,16 @@ def _run(operation, dem_path, horizons_path, output_path, *, sun_vectors_m, time
             output_transform=output_transform)
         progress.completed = len(product.completed)
         progress("start", progress.completed, len(patches))
-        pending = [p for p in patches if not product.is_complete(p.tile_y, p.tile_x)]
-        by_key = {product.key(p.tile_y, p.tile_x): p for p in pending}
-        encode_workers = min(8, os.cpu_count() or 1) if bands > 1 else 0
-        with product.batch_writer(int(durable_batch_size), encode_workers=encode_workers) as batch:
-
-            def durable(keys):
-                for key in keys:
-                    patch = by_key[key]
-                    progress.completed += 1
-                    progress(product.completed[key], progress.completed, len(patches),
-                             patch.tile_y, patch.tile_x)
-
-            def write(patch, blocks):
-                progress("write", progress.completed, len(patches), patch.tile_y, patch.tile_x)
-                if blocks is None:
-                    durable(batch.write_invalid_patch(patch.tile_y, patch.tile_x, patch.height,
-                                                      patch.width, int(invalid_value)))
-                else:
-                    durable(batch.write_patch(patch.tile_y, patch.tile_x, blocks))
-
-            def finish():
-                durable(batch.checkpoint())
-
-            try:
-                if pipeline_mode == "serial":
-                    for patch in pending:
-                        check_cancel()
-                        _, tile = read(patch)
-                        write(patch, compute(patch, tile))
-                    finish()
-                else:
-                    _Overlap(pending, read, compute, write, finish,
-                             reader_workers=int(reader_worker_count),
-                             decoded_capacity=int(decoded_horizon_capacity),
-                             writer_capacity=int(writer_queue_capacity)).run()
-            finally:
-                durable(batch.checkpoint())  # whole patches written before an error stay
+        drive_patches(product, patches, store, observer_height_m,
+                      compute=lambda patch, tile: (block for _, block in
+                                                   per_patch(calculator, dem, tile, prepared,
+                                                             patch)),
+                      progress=progress, check_cancel=check_cancel,
+                      invalid_fill=int(invalid_value), pipeline_mode=pipeline_mode,
+                      reader_worker_count=reader_worker_count,
+                      decoded_horizon_capacity=decoded_horizon_capacity,
+                      writer_queue_capacity=writer_queue_capacity,
+                      durable_batch_size=durable_batch_size)
         check_cancel()
         path = product.finalize()
         product = None

@@ -323,3 +323,24 @@ def test_an_unreadable_tile_is_an_invalid_patch(site, tmp_path):
     mask, _ = geotiff.read_geotiff(out)
     assert np.all(mask[0:128, 128:256] == 5)
     assert set(np.unique(mask[0:128, 0:128])) <= {0, 255}
+
+
+@pytest.mark.parametrize("stage", ["read", "calculate", "write", "valid"])
+def test_a_failing_callback_in_any_stage_ends_the_bounded_run(site, tmp_path, stage):
+    """The first error of any pipeline thread stops the others and is raised as
+    is; the staged product stays for a resume."""
+    dem, dem_path, horizons, written, sun = site
+
+    class Boom(RuntimeError):
+        pass
+
+    def on_event(event):
+        if event.stage == stage:
+            raise Boom(stage)
+
+    with pytest.raises(Boom):
+        generate_psr(dem_path, horizons, tmp_path / f"{stage}.tif", sun_vectors_m=sun,
+                     backend="cpu", progress_event_callback=on_event, reader_worker_count=2,
+                     decoded_horizon_capacity=2)
+    assert not (tmp_path / f"{stage}.tif").exists()
+    assert len(list(tmp_path.glob(f".{stage}.tif.lunarscout-partial.*"))) == 3
