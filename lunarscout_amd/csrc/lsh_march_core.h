@@ -154,8 +154,13 @@ LSH_HD T lsh_min(T a, T b) {
   return sizeof(T) == 4 ? static_cast<T>(__builtin_fminf(static_cast<float>(a), static_cast<float>(b)))
                         : static_cast<T>(__builtin_fmin(static_cast<double>(a), static_cast<double>(b)));
 }
+// |a| (a source modifier on the device: no instruction of its own); the cell
+// exit below never divides by a zero it would sign
 template <class T>
-LSH_HD T lsh_abs(T a) { return a < T(0) ? -a : a; }
+LSH_HD T lsh_abs(T a) {
+  return sizeof(T) == 4 ? static_cast<T>(__builtin_fabsf(static_cast<float>(a)))
+                        : static_cast<T>(__builtin_fabs(static_cast<double>(a)));
+}
 
 // Elevation slope, on a sphere of radius R, of a point hd metres above an observer
 // at height oz and `distance` metres away (cuda_backend.py:777-795, 812-830).
