@@ -162,6 +162,79 @@ LSH_HD T lsh_abs(T a) {
                         : static_cast<T>(__builtin_fabs(static_cast<double>(a)));
 }
 
+// True in every lane when the predicate holds in any active lane of the
+// wavefront (a wave-uniform branch condition); one lane on the host.
+LSH_HD bool lsh_any_lane(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_ballot_w64(p) != 0;
+#else
+  return p;
+#endif
+}
+
+// Correctly rounded double square root and quotient for the operands the march
+// forms. On the device these are the steps of the compiler's IEEE sequences --
+// reciprocal-square-root or reciprocal estimate, Goldschmidt / Newton
+// refinement, a final residual correction -- without the operand scaling those
+// sequences apply only when an exponent nears the ends of the double range (a
+// radicand below 2^-767; a quotient whose operands lie 768 binades apart, or
+// whose numerator is below 2^-969 or zero) and without the special-value
+// fix-up (zero, infinity, NaN). For the operands below the results equal the
+// host's sqrt and division bit for bit.
+//
+// lsh_sqrt_normal: finite x >= 2^-767.
+LSH_HD double lsh_sqrt_normal(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double h = y * 0.5;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, h, g);
+#else
+  return __builtin_sqrt(x);
+#endif
+}
+// sqrt(x) for any x >= 0: the sequence above unless a lane of the wavefront
+// holds a radicand below 2^-767 (then the general one, for every lane).
+LSH_HD double lsh_sqrt_any(double x) {
+  if (lsh_any_lane(!(x >= 0x1p-767))) return __builtin_sqrt(x);
+  return lsh_sqrt_normal(x);
+}
+// lsh_quotient: a / b for 0 <= a < 2^1000 (a = +0 or a >= 2^-969) and
+// 2^-200 < b < 2^200, as the march's cell exits are. A zero numerator gives +0.
+LSH_HD double lsh_quotient(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  const double q = a * r;
+  e = __builtin_fma(-b, q, a);
+  return __builtin_fma(e, r, q);
+#else
+  return a / b;
+#endif
+}
+// a / b with the sign of zero and every special operand as IEEE division (the
+// sequence's final fix-up kept), for normal b whose quotient with a stays
+// normal or zero: the slopes and step lengths of the march.
+LSH_HD double lsh_ratio(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_div_fixup(lsh_quotient(a, b), b, a);
+#else
+  return a / b;
+#endif
+}
+LSH_HD float lsh_quotient(float a, float b) { return a / b; }
+LSH_HD float lsh_ratio(float a, float b) { return a / b; }
+LSH_HD float lsh_sqrt_any(float x) { return __builtin_sqrtf(x); }
+
 // Elevation slope, on a sphere of radius R, of a point hd metres above an observer
 // at height oz and `distance` metres away (cuda_backend.py:777-795, 812-830).
 template <class V>
@@ -247,12 +320,23 @@ struct RayDistance {
 // Distance along the local tangent to the exit of the current cell, in km
 // (cuda_backend.py:733-759): min over the axes of (far edge - p) / d, where an
 // axis with |d| <= 1e-8 uses the inverse 1e30. The axes are compared by
-// cross-multiplication, then the winner takes one division.
+// cross-multiplication, then the winner takes one division. ex0 = min edge - p
+// lies in (-scale, 0], so the far edge's offset is ex0 + scale >= 0 for a
+// positive tangent and ex0 <= 0 for a negative one: |offset| / |d| is the time
+// to it. A wavefront in which no lane has a tiny tangent (the rule) takes that
+// form; otherwise every lane takes the general one, which gives the same
+// quotient for the lanes without a tiny tangent.
 template <class S>
 LSH_HD S lsh_cell_exit(S ex0, S ey0, S scale, S dx, S dy) {
   const S adx = lsh_abs(dx), ady = lsh_abs(dy);
   const bool tiny_x = !(adx > S(1e-8)), tiny_y = !(ady > S(1e-8));
-  // (edge - p) for the edge the tangent points at; ex0 = min edge - p <= 0
+  if (!lsh_any_lane(tiny_x || tiny_y)) {
+    const S nx = lsh_abs(ex0 + (dx > S(0) ? scale : S(0)));
+    const S ny = lsh_abs(ey0 + (dy > S(0) ? scale : S(0)));
+    const bool use_x = nx * ady <= ny * adx;
+    return lsh_quotient(use_x ? nx : ny, use_x ? adx : ady);
+  }
+  // (edge - p) for the edge the tangent points at
   const S ax = tiny_x || dx > S(0) ? ex0 + scale : -ex0;
   const S ay = tiny_y || dy > S(0) ? ey0 + scale : -ey0;
   // an axis with a tiny tangent never wins unless both are tiny
@@ -310,7 +394,7 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     const int ix = static_cast<int>(ax), iy = static_cast<int>(ay);
     const bool near = s.below(S(0.5));
     const S plx = (px - g.sx) * res, ply = (py - g.sy) * res;
-    const S planar = lsh_sqrt(plx * plx + ply * ply);
+    const S planar = lsh_sqrt_any(plx * plx + ply * ply);
     const S p2 = planar * planar, p3 = p2 * planar;
     S chord = g.c1 * planar;
     chord = chord + g.c2 * p2;
@@ -411,8 +495,8 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
       const V lz = (hd * (two_radius_oz + hd) - sq) * inv_two_observer_radius;
       const V lxx = sq - lz * lz;
       const V num = near ? hd : lz;
-      const V den = near ? tm : (lxx > V(0) ? lsh_sqrt(lxx) : V(1e-6));
-      V slope = num / den;
+      const V den = near ? tm : (lxx > V(0) ? lsh_sqrt_any(lxx) : V(1e-6));
+      V slope = lsh_ratio(num, den);
       if (!taps_valid) slope = V(-1e30);
       if (trace.sample(sample_index, ix, iy, static_cast<double>(tx), static_cast<double>(ty),
                        static_cast<double>(hd), static_cast<double>(tm), near))
@@ -424,8 +508,8 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
       if (kCount) ++cnt.samples;
       ++sample_index;
       // adaptive level-0 advance (cuda_backend.py:833-861)
-      const S mag = lsh_sqrt(dx * dx + dy * dy);
-      const S pixel_step = mag > S(1e-6) ? S(1) / mag : S(0.0005);
+      const S mag = lsh_sqrt_any(dx * dx + dy * dy);
+      const S pixel_step = mag > S(1e-6) ? lsh_ratio(S(1), mag) : S(0.0005);
       const S margin = static_cast<S>(current - slope);
       const S margin_step = margin > S(0) ? ((margin * true_m) * S(1.732)) * S(0.001) : S(0);
       const S angular_step = (true_m * S(0.00151)) * S(0.001);
