@@ -172,6 +172,15 @@ LSH_HD bool lsh_any_lane(bool p) {
 #endif
 }
 
+// An empty statement the compiler may not move or remove: placed in the rare
+// side of a wave-uniform branch, it keeps the branch a branch (the compiler
+// would otherwise evaluate both sides for every lane and select).
+LSH_HD void lsh_keep_branch() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __asm__ volatile("" ::: "memory");
+#endif
+}
+
 // Correctly rounded double square root and quotient for the operands the march
 // forms. On the device these are the steps of the compiler's IEEE sequences --
 // reciprocal-square-root or reciprocal estimate, Goldschmidt / Newton
@@ -248,10 +257,16 @@ LSH_HD V lsh_sphere_slope(V hd, V oz, V distance, V radius, V two_observer_radiu
 }
 
 // possible <= current for the cull test (cuda_backend.py:767-802) without
-// forming the quotient: the bound's slope is lz / lx with lx = sqrt(sq - lz^2)
-// (or 1e-6), and the comparison is decided on lz, lx^2 and current^2. Exact
-// arithmetic gives the same answer; rounding can differ only at ties. Every case
-// is evaluated and the answer selected (no branches inside a wavefront).
+// forming the quotient: the bound's slope is lz / lx with lx = sqrt(lxx),
+// lxx = sq - lz^2 (or lx = 1e-6 when lxx <= 0). For lxx > 0,
+// lz / sqrt(lxx) <= current is decided on signed squares:
+// lz |lz| <= current |current| lxx, which holds the cases of the comparison
+// (lz <= 0 with current >= 0 always culls; lz > 0 with current <= 0 never
+// does; otherwise lz^2 against current^2 lxx, the inequality reversed when both
+// are negative). Exact arithmetic gives the same answer; rounding can differ
+// only at ties. A wavefront in which no lane is near the observer and no lane
+// has lxx <= 0 (the rule far out) takes that one comparison; otherwise every
+// lane evaluates the near and the lx = 1e-6 cases as well.
 template <class V>
 LSH_HD bool lsh_bound_cannot_rise(V hd, V distance, bool near, V two_radius_oz,
                                   V inv_two_observer_radius, V current) {
@@ -259,13 +274,11 @@ LSH_HD bool lsh_bound_cannot_rise(V hd, V distance, bool near, V two_radius_oz,
   const V rs = two_radius_oz + hd;  // 2R + h + oz with h = oz + hd
   const V lz = (hd * rs - sq) * inv_two_observer_radius;
   const V lxx = sq - lz * lz;
-  const V rhs = current * current * lxx;
-  const V lz2 = lz * lz;
   const bool degenerate = !(lxx > V(0));
-  const bool below = lz <= V(0);
-  const bool far_result = degenerate ? lz * V(1e6) <= current
-                                     : (below ? (current >= V(0)) | (lz2 >= rhs)
-                                              : (current > V(0)) & (lz2 <= rhs));
+  const bool ordinary = lz * lsh_abs(lz) <= (current * lsh_abs(current)) * lxx;
+  if (!lsh_any_lane(near || degenerate)) return ordinary;
+  lsh_keep_branch();
+  const bool far_result = degenerate ? lz * V(1e6) <= current : ordinary;
   return near ? hd <= current * distance : far_result;  // near: distance >= 1 m
 }
 
@@ -319,26 +332,30 @@ struct RayDistance {
 
 // Distance along the local tangent to the exit of the current cell, in km
 // (cuda_backend.py:733-759): min over the axes of (far edge - p) / d, where an
-// axis with |d| <= 1e-8 uses the inverse 1e30. The axes are compared by
-// cross-multiplication, then the winner takes one division. ex0 = min edge - p
-// lies in (-scale, 0], so the far edge's offset is ex0 + scale >= 0 for a
-// positive tangent and ex0 <= 0 for a negative one: |offset| / |d| is the time
-// to it. A wavefront in which no lane has a tiny tangent (the rule) takes that
-// form; otherwise every lane takes the general one, which gives the same
-// quotient for the lanes without a tiny tangent.
+// axis with |d| <= 1e-8 uses the inverse 1e30. The cell spans [x_lo, x_lo +
+// side) x [y_lo, y_lo + side) in the segment's integer-based coordinates, where
+// the position (px, py) lies; x_lo - px is then exact (both are multiples of
+// px's unit in the last place and |x_lo - px| < side), so the far-edge offset
+// is rounded once whichever way it is formed. The axes are compared by
+// cross-multiplication, then the winner takes one division. A wavefront in
+// which no lane has a tiny tangent (the rule) takes the edge the tangent
+// points at as |edge - p| / |d|; otherwise every lane takes the general form,
+// which gives the same quotient for the lanes without a tiny tangent.
 template <class S>
-LSH_HD S lsh_cell_exit(S ex0, S ey0, S scale, S dx, S dy) {
+LSH_HD S lsh_cell_exit(int x_lo, int y_lo, int side, S px, S py, S dx, S dy) {
   const S adx = lsh_abs(dx), ady = lsh_abs(dy);
   const bool tiny_x = !(adx > S(1e-8)), tiny_y = !(ady > S(1e-8));
+  const bool up_x = dx > S(0), up_y = dy > S(0);
   if (!lsh_any_lane(tiny_x || tiny_y)) {
-    const S nx = lsh_abs(ex0 + (dx > S(0) ? scale : S(0)));
-    const S ny = lsh_abs(ey0 + (dy > S(0) ? scale : S(0)));
+    const S nx = lsh_abs(static_cast<S>(up_x ? x_lo + side : x_lo) - px);
+    const S ny = lsh_abs(static_cast<S>(up_y ? y_lo + side : y_lo) - py);
     const bool use_x = nx * ady <= ny * adx;
     return lsh_quotient(use_x ? nx : ny, use_x ? adx : ady);
   }
-  // (edge - p) for the edge the tangent points at
-  const S ax = tiny_x || dx > S(0) ? ex0 + scale : -ex0;
-  const S ay = tiny_y || dy > S(0) ? ey0 + scale : -ey0;
+  lsh_keep_branch();
+  // (edge - p) for the edge the tangent points at (the far one for a tiny tangent)
+  const S ax = tiny_x || up_x ? static_cast<S>(x_lo + side) - px : px - static_cast<S>(x_lo);
+  const S ay = tiny_y || up_y ? static_cast<S>(y_lo + side) - py : py - static_cast<S>(y_lo);
   // an axis with a tiny tangent never wins unless both are tiny
   const bool use_x = tiny_y ? (tiny_x ? ax <= ay : true) : (tiny_x ? false : ax * ady <= ay * adx);
   const S num = use_x ? ax : ay;
@@ -369,6 +386,8 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   const V two_radius_oz = V(2) * radius + V(2) * oz;
   const S chord0 = g.s_chord * S(1000);
   const S min_step = (S(0.5) * res) * S(0.001);
+  // half the level-0 cell side in km; the level-l fallback is this 4^l (exact)
+  const S fallback0 = (res * S(0.5)) * S(0.001);
   const S far_step = (S(0.8) * res) * S(0.001);
   // the reference's px in [0, W-1) is floor(px) in [0, W-2]
   const S max_x = static_cast<S>(d.width - 2), max_y = static_cast<S>(d.height - 2);
@@ -423,14 +442,12 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     // sizes are ceil(W / 4^l) x ceil(H / 4^l), so the cell of a position in
     // [0, W-2] x [0, H-2] lies inside every level and that test never fires.
     int shift = level * 2;
-    int scale = 1 << shift;
     int cx = ix >> shift, cy = iy >> shift;
     float maxh = lsh_load(d, static_cast<uint32_t>(d.bound_offset + d.level_offset[level] +
                                                    cy * d.level_width[level] + cx));
-    S fallback = (lsh_ldexp(res, shift) * S(0.5)) * S(0.001);
-    S exit_distance = lsh_cell_exit(static_cast<S>(cx * scale - g.bx) - px,
-                                    static_cast<S>(cy * scale - g.by) - py,
-                                    static_cast<S>(scale), dx, dy);
+    S fallback = lsh_ldexp(fallback0, shift);
+    S exit_distance = lsh_cell_exit((cx << shift) - g.bx, (cy << shift) - g.by, 1 << shift, px,
+                                    py, dx, dy);
     bool nodata = maxh < -20000.0f;
     bool culled = lsh_bound_cannot_rise(
         static_cast<V>(maxh) - oz, static_cast<V>(lsh_max(true_m - lsh_ldexp(res, shift), S(1))),
@@ -445,7 +462,6 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
                   float(py + g.by), maxh, 0.f, 0.f, 0.f, 0);
         --level;
         shift = level * 2;
-        scale = 1 << shift;
         cx = ix >> shift;
         cy = iy >> shift;
         maxh = lsh_load(d, static_cast<uint32_t>(d.bound_offset + d.level_offset[level] +
@@ -460,10 +476,9 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
           if (nodata || culled || level != 0) ++cnt.tests;
         }
       } while (!(nodata || culled || level == 0));
-      fallback = (lsh_ldexp(res, shift) * S(0.5)) * S(0.001);
-      exit_distance = lsh_cell_exit(static_cast<S>(cx * scale - g.bx) - px,
-                                    static_cast<S>(cy * scale - g.by) - py,
-                                    static_cast<S>(scale), dx, dy);
+      fallback = lsh_ldexp(fallback0, shift);
+      exit_distance = lsh_cell_exit((cx << shift) - g.bx, (cy << shift) - g.by, 1 << shift, px,
+                                    py, dx, dy);
     }
     const bool sampled = !nodata && !culled;  // at level 0
     // -- cell skip (cull / nodata) and level-0 step, selected per lane
