@@ -313,6 +313,8 @@ int lsh_dem_create(lsh_session* session, const float* elevation, int32_t width, 
   for (int i = 0; i < 6; ++i)
     LSH_REQUIRE(std::isfinite(geo_transform[i]) && std::isfinite(projection[i]),
                 "DEM transform and projection must be finite");
+  // row offsets are formed with 24-bit multiplies in the march
+  LSH_REQUIRE(width < (1 << 24) && height < (1 << 24), "DEM dimensions must be below 2^24");
   Dem d;
   int n = pyramid_layout(width, height, d.levels, lsh::kMaxLevels, &d.mip_count);
   LSH_REQUIRE(n > 0, "DEM too large for the pyramid table");
