@@ -23,8 +23,10 @@
 //     its cull bound (cuda_backend.py:777-795), instead of rounding R + h to
 //     float32 (0.125 m) first. With S = float32 the ray distance is kept as an
 //     unevaluated float32 pair (s_hi + s_lo).
-// Each rearrangement is algebraically exact; decisions, constants and the
-// order in which samples and bound tests are taken are unchanged.
+// Each rearrangement is algebraically exact (Horner forms and fused
+// multiply-adds round fewer times than the reference's separate operations);
+// decisions, constants and the order in which samples and bound tests are taken
+// are unchanged.
 
 #pragma once
 
@@ -272,8 +274,8 @@ LSH_HD bool lsh_bound_cannot_rise(V hd, V distance, bool near, V two_radius_oz,
                                   V inv_two_observer_radius, V current) {
   const V sq = distance * distance;
   const V rs = two_radius_oz + hd;  // 2R + h + oz with h = oz + hd
-  const V lz = (hd * rs - sq) * inv_two_observer_radius;
-  const V lxx = sq - lz * lz;
+  const V lz = lsh_fma(hd, rs, -sq) * inv_two_observer_radius;
+  const V lxx = lsh_fma(-lz, lz, sq);
   const bool degenerate = !(lxx > V(0));
   const bool ordinary = lz * lsh_abs(lz) <= (current * lsh_abs(current)) * lxx;
   if (!lsh_any_lane(near || degenerate)) return ordinary;
@@ -423,12 +425,10 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     const int ix = static_cast<int>(ax), iy = static_cast<int>(ay);
     const bool near = s.below(S(0.5));
     const S plx = (px - g.sx) * res, ply = (py - g.sy) * res;
-    const S planar = lsh_sqrt_any(plx * plx + ply * ply);
-    const S p2 = planar * planar, p3 = p2 * planar;
-    S chord = g.c1 * planar;
-    chord = chord + g.c2 * p2;
-    chord = chord + g.c3 * p3;
-    const S true_m = near ? s.times_1000() : chord0 + chord;
+    const S planar = lsh_sqrt_any(lsh_fma(plx, plx, ply * ply));
+    // chord0 + c1 p + c2 p^2 + c3 p^3 (cuda_backend.py:695-698) in Horner form
+    const S chord_h = lsh_fma(planar, lsh_fma(planar, g.c3, g.c2), g.c1);
+    const S true_m = near ? s.times_1000() : lsh_fma(planar, chord_h, chord0);
     // start level: the highest l in 1..top whose cell side 4^l res (formed
     // exactly as res 2^(2l)) fits the beam footprint, else 0. With
     // footprint = f 2^ef and res = r 2^er (f, r in [0.5, 1)), log2(footprint / res)
@@ -517,8 +517,8 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
       const V tm = static_cast<V>(true_m);
       // slope: (h - oz) / d near the observer, lz / lx on the sphere (cuda_backend.py:806-830)
       const V sq = tm * tm;
-      const V lz = (hd * (two_radius_oz + hd) - sq) * inv_two_observer_radius;
-      const V lxx = sq - lz * lz;
+      const V lz = lsh_fma(hd, two_radius_oz + hd, -sq) * inv_two_observer_radius;
+      const V lxx = lsh_fma(-lz, lz, sq);
       const V num = near ? hd : lz;
       const V den = near ? tm : (lxx > V(0) ? lsh_sqrt_any(lxx) : V(1e-6));
       V slope = lsh_ratio(num, den);

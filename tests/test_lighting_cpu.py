@@ -196,3 +196,30 @@ def test_host_lighting_reads_either_layout_to_the_same_bits():
     for k in names:
         np.testing.assert_array_equal(got[0][k], got[1][k], err_msg=k)
     assert 0.0 < float(got[0]["sun_fraction"].mean()) < 1.0
+
+
+@pytest.mark.parametrize("bins", [720, 1440, 2880])
+def test_disk_walk_stays_in_its_row_for_any_bin_count(bins):
+    """The slice walk of an N-bin row reads only that row, at every azimuth.
+
+    Every pixel's row is a constant -10 degree horizon and the rows either side
+    of the sampled pixels hold +80 degrees, so a read past the row's end (the
+    wrap past 359.x deg) or before its start would darken a Sun at 0 degrees
+    elevation. With the rows in bounds the disk is fully lit at every azimuth,
+    for N = 720, 1440 and 2880 alike.
+    """
+    dem = DemGrid(np.zeros((128, 128), np.float32),
+                  np.asarray((0.0, 1.0, 0.0, 15162.0 + 64.0, 0.0, -1.0), np.float64),
+                  ProjectionParameters.south_polar())
+    rows = np.full((128, 128, bins), 80.0, np.float32)
+    pixels = [(5, 7), (64, 64), (127, 126)]
+    for y, x in pixels:
+        rows[y, x] = -10.0
+    az = np.concatenate([np.linspace(0.0, 0.5, 6), np.linspace(359.4, 359.99, 8),
+                         np.linspace(10.0, 350.0, 9)]).astype(np.float32)
+    cpu = CpuLightingSession(threads=2)
+    for y, x in pixels:
+        vectors = np.stack([local_direction_vector(dem, y, x, float(a), 0.0, 1.5e11) for a in az])
+        fractions = np.stack(list(cpu.iter_patch_fraction_tiles(dem, rows, vectors, tile_y=0,
+                                                                tile_x=0)))
+        np.testing.assert_array_equal(fractions[:, y, x], np.ones(az.size, np.float32))
