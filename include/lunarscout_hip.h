@@ -103,7 +103,10 @@ int lsh_event_destroy(void* event);
  * elevation: host float32[height][width]; geo_transform: 6 float64
  * (origin_x, t1, t2, origin_y, t4, t5); projection: 6 float64 (radius,
  * lat0, lon0, scale, false_easting, false_northing). mips_host (nullable):
- * a cached pyramid payload (pyramid.py:34-48) used instead of building. */
+ * a cached pyramid payload (pyramid.py:34-48) used instead of building.
+ * LSH_EINVAL when a side reaches 2^24 or when level 0, the mips and the
+ * per-cell bound copy together exceed 4 GiB (the march addresses them with
+ * 32-bit offsets). */
 int lsh_dem_create(lsh_session* session, const float* elevation, int32_t width, int32_t height,
                    const double* geo_transform, const double* projection,
                    const float* mips_host, int32_t* dem_id);
