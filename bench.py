@@ -21,7 +21,9 @@ kernel's counting build on timed tiles, over the HIP-event time), the lighting
 roofline of the 365-day pass, the batched persistent path on the 48-site
 workload in one launch against the per-tile launches, the end-to-end
 ``generate_horizons`` path with compressed tile writes, the adjoint, and the
-CPU baselines.
+CPU baselines. The CPU leg also checks one timed full-site tile and one batched
+landing-site tile against the float64 oracle (64 px x 1440 az each) and writes
+the errors into the horizon and batched sections.
 
 Prints one JSON line.
 """
@@ -259,8 +261,11 @@ def main() -> int:
         },
         "roofline": roofline,
     }
+    checks = {"horizon": (dems, resident[0], cpu_light_tile)}
+    result["horizon"]["max_err_deg_vs_oracle"] = None  # filled by the CPU leg's oracle check
     if args.batch_tiles > 0:
-        result["batched"] = _batched_vs_plain(session, dems[1], args.batch_tiles)
+        checks["batched"], result["batched"] = _batched_vs_plain(session, dems[1],
+                                                                 args.batch_tiles)
     if args.pipeline_tiles > 0:
         corner = [(c * 128, r * 128) for r in range(0, (args.pipeline_tiles + 3) // 4)
                   for c in range(4)][:args.pipeline_tiles]
@@ -273,6 +278,8 @@ def main() -> int:
     if args.cpu_sample_pixels > 0:
         result["cpu_baseline"] = _cpu_baseline(dems, args.cpu_sample_pixels, cpu_light_tile,
                                                resident[0])
+        for name, error in _oracle_checks(checks).items():
+            result[name]["max_err_deg_vs_oracle"] = error
     print(json.dumps(result), flush=True)
     return 0
 
@@ -463,7 +470,6 @@ def _adjoint_leg(session):
 def _batched_vs_plain(session, regional, tile_count):
     """48 landing sites sharing the regional DEM: per-tile launches against one
     persistent launch (hipGraph, autotuned shape) over the same tiles."""
-    import oracle
     from lunarscout_amd import synthetic
     from lunarscout_amd._lib import HORIZON_DEGREES, DeviceBuffer, Event
     from lunarscout_amd.batch import HorizonBatch, Site, autotune
@@ -521,24 +527,16 @@ def _batched_vs_plain(session, regional, tile_count):
     for slot in range(n):
         a, b = plain.slot(slot).to_host(), out.slot(slot).to_host()
         identical += int(np.all((a == b) | (np.isneginf(a) & np.isneginf(b))))
-    # the batched result against the float64 oracle on 64 pixels of the first sampled tile
+    # the first timed tile's batched degrees, for the oracle check of the CPU leg
     site, c, r = sample[0]
-    dl = [site_dems[site], regional]
-    px = np.linspace(0, PIXELS - 1, 64).astype(np.int32)
-    segs = oracle.segments(dl, tile_column=c, tile_row=r, tile_width=128, azimuth_count=AZIMUTHS)
-    ref, _ = oracle.horizon_tile([oracle.OraclePyramid(d) for d in dl], segs, tile_column=c,
-                                 tile_row=r, pixels=px, float64=True)
-    got = out.slot(0).to_host()[px].astype(np.float64)
-    ref_deg = np.degrees(np.arctan(ref))
-    finite = np.isfinite(ref_deg)
-    err = float(np.max(np.abs(got - ref_deg)[finite])) if finite.any() else 0.0
+    check = ([site_dems[site], regional], (c, r), out.slot(0).to_host())
     batch.close()
     out.free()
     plain.free()
     if segments is not None:
         segments.free()
     busy = kernel_ms / plain_ms
-    return {
+    return check, {
         "workload": "48 landing-site patches (tile counts log-uniform 1..64, 5-20 m/px, seeds "
                     "100-147, centres within +-60 km) sharing the 4096^2 @60 m regional DEM; "
                     f"{len(work)} tiles, {n} timed (evenly spaced over the sites)",
@@ -552,7 +550,7 @@ def _batched_vs_plain(session, regional, tile_count):
                       "trials": [(t["shape"], t["workgroups"], round(t["ms"], 1))
                                  for t in tune["trials"]]},
         "bit_identical_tiles": f"{identical}/{n}",
-        "max_err_deg_vs_oracle": {"value": err, "sample": "64 px x 1440 az of the first timed tile"},
+        "max_err_deg_vs_oracle": None,  # filled by the CPU leg's oracle check
     }
 
 
@@ -616,6 +614,30 @@ def _products_leg(primary, tiles, root):
                      "bytes_written": path.stat().st_size}
         path.unlink()
         log(f"{name} product: {len(tiles)} patches in {wall:.2f} s")
+    return out
+
+
+def _oracle_checks(checks, pixels: int = 64) -> dict:
+    """Max |GPU - float64 oracle| in degrees on ``pixels`` pixels x 1440 az of one
+    timed tile per leg: {leg: {"value", "within_1e-3_deg", "sample"}}."""
+    import oracle
+
+    out = {}
+    px = np.linspace(0, PIXELS - 1, pixels).astype(np.int32)
+    for name, (dems, (c, r), degrees) in checks.items():
+        segs = oracle.segments(dems, tile_column=c, tile_row=r, tile_width=128,
+                               azimuth_count=AZIMUTHS)
+        ref, _ = oracle.horizon_tile([oracle.OraclePyramid(d) for d in dems], segs, tile_column=c,
+                                     tile_row=r, pixels=px, float64=True)
+        ref_deg = np.degrees(np.arctan(ref))
+        got = degrees[px].astype(np.float64)
+        finite = np.isfinite(ref_deg)
+        delta = np.abs(got - ref_deg)[finite]
+        out[name] = {"value": float(delta.max()) if delta.size else 0.0,
+                     "within_1e-3_deg": float(np.mean(delta <= 1e-3)) if delta.size else 1.0,
+                     "horizonless_bins_differing": int(np.count_nonzero(finite != np.isfinite(got))),
+                     "sample": f"{pixels} px x {AZIMUTHS} az of tile ({c}, {r})"}
+        log(f"oracle check {name}: max |delta| {out[name]['value']:.3g} deg")
     return out
 
 
