@@ -69,6 +69,7 @@ struct MarchDem {
   int32_t all_valid;
   double resolution;        // map_resolution widened
   int32_t resolution_exponent;  // its binary exponent (frexp)
+  double resolution_mantissa;   // and its mantissa in [0.5, 1)
   int32_t level_offset[kMaxLevels];  // element offsets from level0 (level 0: 0)
   int32_t level_width[kMaxLevels];
   int32_t level_height[kMaxLevels];

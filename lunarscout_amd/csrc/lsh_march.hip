@@ -86,7 +86,7 @@ __device__ __attribute__((always_inline)) void march_pixel(
         primary.column_resolution / d.column_resolution);
     MarchView view{d.level0, d.buffer_bytes, d.bound_offset, s_off[pass], s_w[pass], s_h[pass],
                    d.width, d.height, d.level_count, d.map_resolution, d.radius, d.all_valid,
-                   d.resolution, d.resolution_exponent};
+                   d.resolution, d.resolution_exponent, d.resolution_mantissa};
     lsh_march_pass<double, V, kCap, kCount>(g, view, oz, pass, current, best, cnt);
   }
   float result = current > V(-1e29) ? static_cast<float>(current) : -INFINITY;
@@ -244,7 +244,7 @@ MarchDem march_dem(const Dem& dem) {
   m.column_resolution = column_res;
   m.map_resolution = (column_res + row_res) * 0.5f;
   m.resolution = static_cast<double>(m.map_resolution);
-  (void)std::frexp(m.resolution, &m.resolution_exponent);
+  m.resolution_mantissa = std::frexp(m.resolution, &m.resolution_exponent);
   m.radius = static_cast<float>(dem.projection[0]);
   m.all_valid = dem.all_valid;
   const int64_t level0_floats = static_cast<int64_t>(dem.width) * dem.height;

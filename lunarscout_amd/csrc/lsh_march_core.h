@@ -60,6 +60,7 @@ struct MarchView {
   // [0.5, 1)): uniform values the loop reads from scalar registers
   double resolution;
   int32_t resolution_exponent;
+  double resolution_mantissa;  // res = resolution_mantissa 2^resolution_exponent
 };
 
 // One pass's interpolated segment with its start relative to (bx, by), in the
@@ -135,6 +136,13 @@ LSH_HD T lsh_ldexp(T x, int e) {
   return sizeof(T) == 4 ? static_cast<T>(__builtin_ldexpf(static_cast<float>(x), e))
                         : static_cast<T>(__builtin_ldexp(static_cast<double>(x), e));
 }
+// x = f 2^e with f in [0.5, 1): returns f, sets e
+template <class T>
+LSH_HD T lsh_frexp(T x, int& e) {
+  e = 0;
+  return sizeof(T) == 4 ? static_cast<T>(__builtin_frexpf(static_cast<float>(x), &e))
+                        : static_cast<T>(__builtin_frexp(static_cast<double>(x), &e));
+}
 // the binary exponent e of x = f 2^e, f in [0.5, 1)
 template <class T>
 LSH_HD int lsh_exponent(T x) {
@@ -180,6 +188,17 @@ LSH_HD bool lsh_any_lane(bool p) {
 LSH_HD void lsh_keep_branch() {
 #if defined(__HIP_DEVICE_COMPILE__)
   __asm__ volatile("" ::: "memory");
+#endif
+}
+
+// The value unchanged, but opaque to the compiler (no instruction): tests formed
+// from it are not merged with tests of the original.
+template <class T>
+LSH_HD void lsh_keep_value(T& x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __asm__ volatile("" : "+v"(x));
+#else
+  (void)x;
 #endif
 }
 
@@ -364,14 +383,20 @@ LSH_HD S lsh_cell_exit(int x_lo, int y_lo, int side, S px, S py, S dx, S dy) {
     const bool use_x = nx * ady <= ny * adx;
     return lsh_quotient(use_x ? nx : ny, use_x ? adx : ady);
   }
-  lsh_keep_branch();
+  // the tests formed again from the same values inside this branch, so that
+  // their other polarity is not computed for every position
+  S ox = dx, oy = dy;
+  lsh_keep_value(ox);
+  lsh_keep_value(oy);
+  const S gx = lsh_abs(ox), gy = lsh_abs(oy);
+  const bool tx = !(gx > S(1e-8)), ty = !(gy > S(1e-8));
   // (edge - p) for the edge the tangent points at (the far one for a tiny tangent)
-  const S ax = tiny_x || up_x ? static_cast<S>(x_lo + side) - px : px - static_cast<S>(x_lo);
-  const S ay = tiny_y || up_y ? static_cast<S>(y_lo + side) - py : py - static_cast<S>(y_lo);
+  const S ax = tx || dx > S(0) ? static_cast<S>(x_lo + side) - px : px - static_cast<S>(x_lo);
+  const S ay = ty || dy > S(0) ? static_cast<S>(y_lo + side) - py : py - static_cast<S>(y_lo);
   // an axis with a tiny tangent never wins unless both are tiny
-  const bool use_x = tiny_y ? (tiny_x ? ax <= ay : true) : (tiny_x ? false : ax * ady <= ay * adx);
+  const bool use_x = ty ? (tx ? ax <= ay : true) : (tx ? false : ax * gy <= ay * gx);
   const S num = use_x ? ax : ay;
-  const S den = tiny_x && tiny_y ? S(1e-30) : (use_x ? adx : ady);
+  const S den = tx && ty ? S(1e-30) : (use_x ? gx : gy);
   return num / den;
 }
 
@@ -400,6 +425,7 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   const S min_step = (S(0.5) * res) * S(0.001);
   // half the level-0 cell side in km; the level-l fallback is this 4^l (exact)
   const S fallback0 = (res * S(0.5)) * S(0.001);
+  const S res_mantissa = static_cast<S>(d.resolution_mantissa);
   const S far_step = (S(0.8) * res) * S(0.001);
   // the reference's px in [0, W-1) is floor(px) in [0, W-2]
   const S max_x = static_cast<S>(d.width - 2), max_y = static_cast<S>(d.height - 2);
@@ -429,15 +455,17 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     // chord0 + c1 p + c2 p^2 + c3 p^3 (cuda_backend.py:695-698) in Horner form
     const S chord_h = lsh_fma(planar, lsh_fma(planar, g.c3, g.c2), g.c1);
     const S true_m = near ? s.times_1000() : lsh_fma(planar, chord_h, chord0);
-    // start level: the highest l in 1..top whose cell side 4^l res (formed
-    // exactly as res 2^(2l)) fits the beam footprint, else 0. With
-    // footprint = f 2^ef and res = r 2^er (f, r in [0.5, 1)), log2(footprint / res)
-    // lies in (ef - er - 1, ef - er + 1), so that l is floor((ef - er + 1) / 2)
-    // or one less; one exact comparison decides. No loop, no branch.
+    // start level: the highest l in 1..top whose cell side 4^l res fits the beam
+    // footprint, else 0. With footprint = f 2^ef and res = r 2^er (f, r in
+    // [0.5, 1)), floor(log2(footprint / res)) = ef - er - (f < r), exactly, and
+    // that l is its floor half, clamped to [0, top] (negative exactly when
+    // footprint < res). No loop, no branch, no rounding.
     const S footprint = true_m * static_cast<S>(kBeamWidth);
-    int level = (lsh_exponent(footprint) - d.resolution_exponent + 1) >> 1;
-    if (!(lsh_ldexp(res, 2 * level) <= footprint)) --level;
-    level = footprint >= res ? (level < top_level ? level : top_level) : 0;
+    int footprint_exponent;
+    const S footprint_mantissa = lsh_frexp(footprint, footprint_exponent);
+    int level = (footprint_exponent - d.resolution_exponent -
+                 (footprint_mantissa < res_mantissa ? 1 : 0)) >> 1;
+    level = level < 0 ? 0 : (level < top_level ? level : top_level);
 
     // -- hierarchy tests at this position (cuda_backend.py:706-868)
     // hierarchy_bilinear_bound (cuda_backend.py:47-60) -- the maximum of the

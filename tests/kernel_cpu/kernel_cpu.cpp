@@ -31,6 +31,11 @@ int res_exponent(float res) {
   (void)std::frexp(double(res), &e);
   return e;
 }
+
+double res_mantissa(float res) {
+  int e = 0;
+  return std::frexp(double(res), &e);
+}
 }  // namespace
 
 extern "C" {
@@ -78,7 +83,8 @@ float ray(const kc_pyramid* pyrs, int D, const float* segments, int C, int az, i
                      t.off[p],
                      t.lw[p], t.lh[p], pyrs[p].dem.width, pyrs[p].dem.height,
                      pyrs[p].level_count, t.mapres[p], float(pyrs[p].dem.projection[0]),
-                     t.all_valid[p], double(t.mapres[p]), res_exponent(t.mapres[p])};
+                     t.all_valid[p], double(t.mapres[p]), res_exponent(t.mapres[p]),
+                     res_mantissa(t.mapres[p])};
     if (cap)
       lsh::lsh_march_pass<S, V, true, true>(g, v, oz, p, current, best, cnt, tr);
     else
@@ -254,7 +260,8 @@ int kcpu_march_segment(const float* segment, const kc_pyramid* pyr, double obser
   lsh::MarchView v{t.combined[0].data(), uint32_t(4 * t.combined[0].size()), t.bound_offset[0],
                    t.off[0], t.lw[0], t.lh[0], pyr->dem.width, pyr->dem.height,
                    pyr->level_count, t.mapres[0], float(pyr->dem.projection[0]),
-                   t.all_valid[0], double(t.mapres[0]), res_exponent(t.mapres[0])};
+                   t.all_valid[0], double(t.mapres[0]), res_exponent(t.mapres[0]),
+                   res_mantissa(t.mapres[0])};
   lsh::MarchCounters cnt{0, 0, 0};
   int n = 0;
   RowTrace tr{{}, rows, cap, &n};
