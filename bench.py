@@ -9,11 +9,15 @@ passes, then from those horizons the lightmap (finite-disk sun fraction bytes
 per time), the Earth-visibility fraction and the PSR mask for 125 Sun and 125
 Earth Moon-ME vectors at 6-hour steps over 31 days.
 
-A step is one batch of tiles (``--tiles-per-step``, default 24): FP64 segment
+A step is one batch of tiles (``--tiles-per-step``, default 32): FP64 segment
 generation, the fused two-pass march into HBM (degrees), and the streamed
 lighting pass over the batch. Tiles are taken in the order i * 97 mod 256, so
-any run of consecutive steps samples rows and columns of the whole site evenly.
-DEMs and vectors are resident in HBM before the timed region.
+any run of consecutive steps samples rows and columns of the whole site evenly;
+the default 8 timed steps march every one of the 256 tiles once. Each marched
+tile keeps its own slot in HBM (24.2 GB for the whole site, azimuth-major), so
+the year of lighting (4380 Sun + 4380 Earth vectors and the PSR test) runs over
+all 256 tiles in one launch. DEMs and vectors are resident in HBM before the
+timed region.
 
 Besides the headline the line carries: the march's roofline (FP64 VALU: the
 method's FLOPs per hierarchy test and per level-0 sample, counted by the
@@ -70,9 +74,9 @@ def log(*args):
 def parse_args():
     p = argparse.ArgumentParser(description=__doc__)
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=4)
+    p.add_argument("--steps", type=int, default=8)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--tiles-per-step", type=int, default=24)
+    p.add_argument("--tiles-per-step", type=int, default=32)
     p.add_argument("--cpu-sample-pixels", type=int, default=256,
                    help="pixels of one tile the CPU baseline marches (0 disables it)")
     p.add_argument("--counter-tiles", type=int, default=4,
@@ -129,8 +133,14 @@ def main() -> int:
     tiles = [grid[(i * 97) % len(grid)] for i in range(len(grid))]
     per_step = args.tiles_per_step
     major = not args.pixel_major
-    horizons = DeviceBuffer.empty(session, (per_step, AZIMUTHS, PIXELS) if major
-                                  else (per_step, PIXELS, AZIMUTHS), np.float32)
+    # every marched tile keeps its own slot in HBM (all 256 with the defaults:
+    # 24.2 GB); the i-th tile marched goes to slot i mod slots
+    slots = min(len(tiles) // per_step, args.warmup + args.steps) * per_step
+    if slots < per_step:
+        raise SystemExit("--tiles-per-step must not exceed the site's 256 tiles")
+    horizons = DeviceBuffer.empty(session, (slots, AZIMUTHS, PIXELS) if major
+                                  else (slots, PIXELS, AZIMUTHS), np.float32)
+    resident = {}  # slot -> the tile its horizons belong to
     segments = None
     outputs = None
     stats = {"march_ms": 0.0, "march_launches": 0, "segments_ms": 0.0, "lighting_ms": 0.0,
@@ -139,18 +149,21 @@ def main() -> int:
     def step(index: int, record: bool):
         nonlocal segments, outputs
         batch = [tiles[(index * per_step + i) % len(tiles)] for i in range(per_step)]
+        first = (index * per_step) % slots
         events = []
-        for slot, (c, r) in enumerate(batch):
+        for i, (c, r) in enumerate(batch):
             e0 = Event(session).record()
             segments = session.segments_device(dems, tile_column=c, tile_row=r, out=segments)
             e1 = Event(session).record()
             session.march_device(dems, segments, tile_column=c, tile_row=r,
-                                 flags=HORIZON_DEGREES, output=horizons.slot(slot),
+                                 flags=HORIZON_DEGREES, output=horizons.slot(first + i),
                                  azimuth_major=major)
             e2 = Event(session).record()
             events.append((e0, e1, e2))
+            resident[first + i] = (c, r)
         l0 = Event(session).record()
-        outputs = lighting.lighting_tiles(primary, batch, horizons, sun=sun_d, earth=earth_d,
+        outputs = lighting.lighting_tiles(primary, batch, horizons.slots(first, per_step),
+                                          sun=sun_d, earth=earth_d,
                                           psr_vectors=psr_d, sun_bytes=True, sun_mean=True,
                                           earth_visible=True, psr=True, outputs=outputs,
                                           azimuth_major=major)
@@ -179,8 +192,7 @@ def main() -> int:
     march_ms = stats["march_ms"] / max(stats["march_launches"], 1)
     lighting_ms = stats["lighting_ms"] / max(stats["lighting_launches"], 1)
     light_px_times = per_step * PIXELS * (len(sun) + len(earth) + len(psr_vectors))
-    last = args.warmup + args.steps - 1
-    resident = [tiles[(last * per_step + i) % len(tiles)] for i in range(per_step)]
+    resident = [resident[i] for i in range(slots)]  # slot order
 
     # traversal counters on timed tiles -> the method's FLOPs and gathered bytes per tile
     counted = None
@@ -243,6 +255,7 @@ def main() -> int:
             "workload": "full site, plain HIP path: 2048^2 @5 m primary + 4096^2 @60 m regional, "
                         "256 tiles x 1440 az, lighting + Earth + PSR from 125 vectors",
             "tiles_per_step": per_step,
+            "resident_tiles": slots,
             "seeds": {"primary": 1, "regional": 2, "sun": 3, "earth": 4},
         },
         "horizon": {

@@ -255,6 +255,21 @@ class DeviceBuffer:
         view.free = lambda: None
         return view
 
+    def slots(self, start: int, count: int) -> "DeviceBuffer":
+        """A view of sub-arrays ``start .. start + count - 1`` along the first axis."""
+        if not (0 <= start and 0 < count and start + count <= self.shape[0]):
+            raise ContractValidationError("slot range out of range")
+        view = DeviceBuffer.__new__(DeviceBuffer)
+        view.session = self.session
+        view.dtype = self.dtype
+        view.shape = (int(count),) + self.shape[1:]
+        per = int(np.prod(self.shape[1:], dtype=np.int64)) * self.dtype.itemsize
+        view.nbytes = per * int(count)
+        view.ptr = self.ptr + int(start) * per
+        view._parent = self
+        view.free = lambda: None
+        return view
+
     def free(self) -> None:
         if self.ptr and self.session.handle:
             load_library().lsh_free(self.session.handle, self.ptr)
