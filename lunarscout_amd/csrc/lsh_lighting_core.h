@@ -84,7 +84,7 @@ LSH_LHD void azimuth_elevation(const Frame& f, const double* v, double& az, doub
   double a = atan2_finite(e[0], e[1]) * (180.0 / kPi);
   if (a < 0.0) a += 360.0;
   az = a;
-  const double horizontal = __builtin_sqrt(e[0] * e[0] + e[1] * e[1]);
+  const double horizontal = lsh_sqrt_any(e[0] * e[0] + e[1] * e[1]);
   el = atan2_finite(e[2], horizontal) * (180.0 / kPi);
 }
 
@@ -102,8 +102,13 @@ template <class Row>
 LSH_LHD double horizon_at(const Row& hz, int N, double az) {
   double pos = az * (N / 360.0);
   double fl = floor(pos);
-  int lower = static_cast<int>(fl) % N;
-  if (lower < 0) lower += N;
+  // az in [0, 360] puts lower in [0, N]: the remainder (an integer division)
+  // only when a lane of the wavefront is outside [0, N)
+  int lower = static_cast<int>(fl);
+  if (lsh_any_lane(lower < 0 || lower >= N)) {
+    lower %= N;
+    if (lower < 0) lower += N;
+  }
   int upper = lower + 1 == N ? 0 : lower + 1;
   double f = pos - fl;
   double lo = hz[lower];

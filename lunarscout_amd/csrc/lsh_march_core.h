@@ -32,6 +32,8 @@
 
 #include <stdint.h>
 
+#include "lsh_math.h"
+
 #if defined(__HIPCC__)
 #define LSH_HD __host__ __device__ inline
 #define LSH_HD_MEMBER __host__ __device__
@@ -172,98 +174,6 @@ LSH_HD T lsh_abs(T a) {
                         : static_cast<T>(__builtin_fabs(static_cast<double>(a)));
 }
 
-// True in every lane when the predicate holds in any active lane of the
-// wavefront (a wave-uniform branch condition); one lane on the host.
-LSH_HD bool lsh_any_lane(bool p) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __builtin_amdgcn_ballot_w64(p) != 0;
-#else
-  return p;
-#endif
-}
-
-// An empty statement the compiler may not move or remove: placed in the rare
-// side of a wave-uniform branch, it keeps the branch a branch (the compiler
-// would otherwise evaluate both sides for every lane and select).
-LSH_HD void lsh_keep_branch() {
-#if defined(__HIP_DEVICE_COMPILE__)
-  __asm__ volatile("" ::: "memory");
-#endif
-}
-
-// The value unchanged, but opaque to the compiler (no instruction): tests formed
-// from it are not merged with tests of the original.
-template <class T>
-LSH_HD void lsh_keep_value(T& x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  __asm__ volatile("" : "+v"(x));
-#else
-  (void)x;
-#endif
-}
-
-// Correctly rounded double square root and quotient for the operands the march
-// forms. On the device these are the steps of the compiler's IEEE sequences --
-// reciprocal-square-root or reciprocal estimate, Goldschmidt / Newton
-// refinement, a final residual correction -- without the operand scaling those
-// sequences apply only when an exponent nears the ends of the double range (a
-// radicand below 2^-767; a quotient whose operands lie 768 binades apart, or
-// whose numerator is below 2^-969 or zero) and without the special-value
-// fix-up (zero, infinity, NaN). For the operands below the results equal the
-// host's sqrt and division bit for bit.
-//
-// lsh_sqrt_normal: finite x >= 2^-767.
-LSH_HD double lsh_sqrt_normal(double x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  const double y = __builtin_amdgcn_rsq(x);
-  double g = x * y;
-  double h = y * 0.5;
-  const double r = __builtin_fma(-h, g, 0.5);
-  g = __builtin_fma(g, r, g);
-  h = __builtin_fma(h, r, h);
-  double d = __builtin_fma(-g, g, x);
-  g = __builtin_fma(d, h, g);
-  d = __builtin_fma(-g, g, x);
-  return __builtin_fma(d, h, g);
-#else
-  return __builtin_sqrt(x);
-#endif
-}
-// sqrt(x) for any x >= 0: the sequence above unless a lane of the wavefront
-// holds a radicand below 2^-767 (then the general one, for every lane).
-LSH_HD double lsh_sqrt_any(double x) {
-  if (lsh_any_lane(!(x >= 0x1p-767))) return __builtin_sqrt(x);
-  return lsh_sqrt_normal(x);
-}
-// lsh_quotient: a / b for 0 <= a < 2^1000 (a = +0 or a >= 2^-969) and
-// 2^-200 < b < 2^200, as the march's cell exits are. A zero numerator gives +0.
-LSH_HD double lsh_quotient(double a, double b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  double r = __builtin_amdgcn_rcp(b);
-  double e = __builtin_fma(-b, r, 1.0);
-  r = __builtin_fma(r, e, r);
-  e = __builtin_fma(-b, r, 1.0);
-  r = __builtin_fma(r, e, r);
-  const double q = a * r;
-  e = __builtin_fma(-b, q, a);
-  return __builtin_fma(e, r, q);
-#else
-  return a / b;
-#endif
-}
-// a / b with the sign of zero and every special operand as IEEE division (the
-// sequence's final fix-up kept), for normal b whose quotient with a stays
-// normal or zero: the slopes and step lengths of the march.
-LSH_HD double lsh_ratio(double a, double b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __builtin_amdgcn_div_fixup(lsh_quotient(a, b), b, a);
-#else
-  return a / b;
-#endif
-}
-LSH_HD float lsh_quotient(float a, float b) { return a / b; }
-LSH_HD float lsh_ratio(float a, float b) { return a / b; }
-LSH_HD float lsh_sqrt_any(float x) { return __builtin_sqrtf(x); }
 
 // Elevation slope, on a sphere of radius R, of a point hd metres above an observer
 // at height oz and `distance` metres away (cuda_backend.py:777-795, 812-830).
