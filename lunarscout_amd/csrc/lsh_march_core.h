@@ -282,32 +282,41 @@ struct RayDistance {
 // which no lane has a tiny tangent (the rule) takes the edge the tangent
 // points at as |edge - p| / |d|; otherwise every lane takes the general form,
 // which gives the same quotient for the lanes without a tiny tangent.
+// The common form: |edge - p| / |d| toward the edge the tangent points at, exact
+// for every lane whose tangent is not tiny on either axis.
 template <class S>
-LSH_HD S lsh_cell_exit(int x_lo, int y_lo, int side, S px, S py, S dx, S dy) {
+LSH_HD S lsh_cell_exit_common(int x_lo, int y_lo, int side, S px, S py, S dx, S dy) {
   const S adx = lsh_abs(dx), ady = lsh_abs(dy);
-  const bool tiny_x = !(adx > S(1e-8)), tiny_y = !(ady > S(1e-8));
-  const bool up_x = dx > S(0), up_y = dy > S(0);
-  if (!lsh_any_lane(tiny_x || tiny_y)) {
-    const S nx = lsh_abs(static_cast<S>(up_x ? x_lo + side : x_lo) - px);
-    const S ny = lsh_abs(static_cast<S>(up_y ? y_lo + side : y_lo) - py);
-    const bool use_x = nx * ady <= ny * adx;
-    return lsh_quotient(use_x ? nx : ny, use_x ? adx : ady);
-  }
-  // the tests formed again from the same values inside this branch, so that
-  // their other polarity is not computed for every position
+  const S nx = lsh_abs(static_cast<S>(dx > S(0) ? x_lo + side : x_lo) - px);
+  const S ny = lsh_abs(static_cast<S>(dy > S(0) ? y_lo + side : y_lo) - py);
+  const bool use_x = nx * ady <= ny * adx;
+  return lsh_quotient(use_x ? nx : ny, use_x ? adx : ady);
+}
+// The general form, for any tangent; its tests are formed from opaque copies of
+// the tangent so that they are not merged with (and do not add to) the tests of
+// the common path.
+template <class S>
+LSH_HD S lsh_cell_exit_general(int x_lo, int y_lo, int side, S px, S py, S dx, S dy) {
   S ox = dx, oy = dy;
   lsh_keep_value(ox);
   lsh_keep_value(oy);
   const S gx = lsh_abs(ox), gy = lsh_abs(oy);
   const bool tx = !(gx > S(1e-8)), ty = !(gy > S(1e-8));
   // (edge - p) for the edge the tangent points at (the far one for a tiny tangent)
-  const S ax = tx || dx > S(0) ? static_cast<S>(x_lo + side) - px : px - static_cast<S>(x_lo);
-  const S ay = ty || dy > S(0) ? static_cast<S>(y_lo + side) - py : py - static_cast<S>(y_lo);
+  const S ax = tx || ox > S(0) ? static_cast<S>(x_lo + side) - px : px - static_cast<S>(x_lo);
+  const S ay = ty || oy > S(0) ? static_cast<S>(y_lo + side) - py : py - static_cast<S>(y_lo);
   // an axis with a tiny tangent never wins unless both are tiny
   const bool use_x = ty ? (tx ? ax <= ay : true) : (tx ? false : ax * gy <= ay * gx);
   const S num = use_x ? ax : ay;
   const S den = tx && ty ? S(1e-30) : (use_x ? gx : gy);
   return num / den;
+}
+// Either form: the common one in a wavefront where no lane has a tiny tangent.
+template <class S>
+LSH_HD S lsh_cell_exit(int x_lo, int y_lo, int side, S px, S py, S dx, S dy) {
+  const bool tiny = !(lsh_abs(dx) > S(1e-8)) || !(lsh_abs(dy) > S(1e-8));
+  if (!lsh_any_lane(tiny)) return lsh_cell_exit_common(x_lo, y_lo, side, px, py, dx, dy);
+  return lsh_cell_exit_general(x_lo, y_lo, side, px, py, dx, dy);
 }
 
 // hierarchy_maximum (cuda_backend.py:656-874) for one DEM pass. S is the
@@ -394,12 +403,26 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     float maxh = lsh_load(d, static_cast<uint32_t>(d.bound_offset + d.level_offset[level] +
                                                    lsh_mul24(cy, d.level_width[level]) + cx));
     S fallback = lsh_ldexp(fallback0, shift);
-    S exit_distance = lsh_cell_exit((cx << shift) - g.bx, (cy << shift) - g.by, 1 << shift, px,
-                                    py, dx, dy);
+    // the common cell exit (formed while the bound load is in flight) and the
+    // cull test's one comparison; a wavefront in which a lane has a tiny
+    // tangent, is near the observer or has lxx <= 0 takes the general forms of
+    // both, for every lane
+    const int x_lo = (cx << shift) - g.bx, y_lo = (cy << shift) - g.by;
+    S exit_distance = lsh_cell_exit_common(x_lo, y_lo, 1 << shift, px, py, dx, dy);
     bool nodata = maxh < -20000.0f;
-    bool culled = lsh_bound_cannot_rise(
-        static_cast<V>(maxh) - oz, static_cast<V>(lsh_max(true_m - lsh_ldexp(res, shift), S(1))),
-        near, two_radius_oz, inv_two_observer_radius, current);
+    const V hd0 = static_cast<V>(maxh) - oz;
+    const V dist0 = static_cast<V>(lsh_max(true_m - lsh_ldexp(res, shift), S(1)));
+    const V sq0 = dist0 * dist0;
+    const V lz0 = lsh_fma(hd0, two_radius_oz + hd0, -sq0) * inv_two_observer_radius;
+    const V lxx0 = lsh_fma(-lz0, lz0, sq0);
+    bool culled = lz0 * lsh_abs(lz0) <= (current * lsh_abs(current)) * lxx0;
+    const bool tiny = !(lsh_abs(dx) > S(1e-8)) || !(lsh_abs(dy) > S(1e-8));
+    if (lsh_any_lane(tiny || near || !(lxx0 > V(0)))) {
+      lsh_keep_branch();
+      exit_distance = lsh_cell_exit_general(x_lo, y_lo, 1 << shift, px, py, dx, dy);
+      culled = lsh_bound_cannot_rise(hd0, dist0, near, two_radius_oz, inv_two_observer_radius,
+                                     current);
+    }
     if (kCount) {
       ++cnt.steps;
       if (nodata || culled || level != 0) ++cnt.tests;
