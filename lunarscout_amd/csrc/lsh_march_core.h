@@ -32,6 +32,8 @@
 
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "lsh_math.h"
 
 #if defined(__HIPCC__)
@@ -398,30 +400,58 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     // reference also ends a ray whose cell lies outside the level; the level
     // sizes are ceil(W / 4^l) x ceil(H / 4^l), so the cell of a position in
     // [0, W-2] x [0, H-2] lies inside every level and that test never fires.
-    int shift = level * 2;
-    int cx = ix >> shift, cy = iy >> shift;
-    float maxh = lsh_load(d, static_cast<uint32_t>(d.bound_offset + d.level_offset[level] +
-                                                   lsh_mul24(cy, d.level_width[level]) + cx));
-    S fallback = lsh_ldexp(fallback0, shift);
-    // the common cell exit (formed while the bound load is in flight) and the
-    // cull test's one comparison; a wavefront in which a lane has a tiny
-    // tangent, is near the observer or has lxx <= 0 takes the general forms of
-    // both, for every lane
-    const int x_lo = (cx << shift) - g.bx, y_lo = (cy << shift) - g.by;
-    S exit_distance = lsh_cell_exit_common(x_lo, y_lo, 1 << shift, px, py, dx, dy);
-    bool nodata = maxh < -20000.0f;
-    const V hd0 = static_cast<V>(maxh) - oz;
-    const V dist0 = static_cast<V>(lsh_max(true_m - lsh_ldexp(res, shift), S(1)));
-    const V sq0 = dist0 * dist0;
-    const V lz0 = lsh_fma(hd0, two_radius_oz + hd0, -sq0) * inv_two_observer_radius;
-    const V lxx0 = lsh_fma(-lz0, lz0, sq0);
-    bool culled = lz0 * lsh_abs(lz0) <= (current * lsh_abs(current)) * lxx0;
-    const bool tiny = !(lsh_abs(dx) > S(1e-8)) || !(lsh_abs(dy) > S(1e-8));
-    if (lsh_any_lane(tiny || near || !(lxx0 > V(0)))) {
+    // The bound test at the start level, the cell exit and the cull test. The
+    // start level is 0 for every lane of a wavefront out to the distance where
+    // the footprint reaches 4 res (about 55 km at 60 m/px, the rule on these
+    // DEMs): that wavefront takes a copy of the code specialised to level 0 --
+    // the cell is the level-0 cell, its bound sits at row iy, column ix of the
+    // level-0 bound copy (no level table), its side is 1 px, its fallback
+    // fallback0 -- and others the general one.
+    int shift = 0, cx = ix, cy = iy;
+    float maxh = 0.0f;
+    S fallback = fallback0, exit_distance = S(0);
+    bool nodata = false, culled = false;
+    auto test_start_level = [&](auto at_level0) {
+      constexpr bool kLevel0 = decltype(at_level0)::value;
+      shift = kLevel0 ? 0 : level * 2;
+      cx = kLevel0 ? ix : ix >> shift;
+      cy = kLevel0 ? iy : iy >> shift;
+      const int side = kLevel0 ? 1 : 1 << shift;
+      const uint32_t bound_index =
+          kLevel0 ? static_cast<uint32_t>(d.bound_offset) + lsh_mul24(iy, d.width) +
+                        static_cast<uint32_t>(ix)
+                  : static_cast<uint32_t>(d.bound_offset + d.level_offset[level] +
+                                          lsh_mul24(cy, d.level_width[level]) + cx);
+      maxh = lsh_load(d, bound_index);
+      fallback = kLevel0 ? fallback0 : lsh_ldexp(fallback0, shift);
+      const S cell_m = kLevel0 ? res : lsh_ldexp(res, shift);
+      // the common cell exit (formed while the bound load is in flight) and
+      // the cull test's one comparison; a wavefront in which a lane has a tiny
+      // tangent, is near the observer or has lxx <= 0 takes the general forms
+      // of both, for every lane
+      const int x_lo = (kLevel0 ? ix : cx << shift) - g.bx;
+      const int y_lo = (kLevel0 ? iy : cy << shift) - g.by;
+      exit_distance = lsh_cell_exit_common(x_lo, y_lo, side, px, py, dx, dy);
+      nodata = maxh < -20000.0f;
+      const V hd0 = static_cast<V>(maxh) - oz;
+      const V dist0 = static_cast<V>(lsh_max(true_m - cell_m, S(1)));
+      const V sq0 = dist0 * dist0;
+      const V lz0 = lsh_fma(hd0, two_radius_oz + hd0, -sq0) * inv_two_observer_radius;
+      const V lxx0 = lsh_fma(-lz0, lz0, sq0);
+      culled = lz0 * lsh_abs(lz0) <= (current * lsh_abs(current)) * lxx0;
+      const bool tiny = !(lsh_abs(dx) > S(1e-8)) || !(lsh_abs(dy) > S(1e-8));
+      if (lsh_any_lane(tiny || near || !(lxx0 > V(0)))) {
+        lsh_keep_branch();
+        exit_distance = lsh_cell_exit_general(x_lo, y_lo, side, px, py, dx, dy);
+        culled = lsh_bound_cannot_rise(hd0, dist0, near, two_radius_oz, inv_two_observer_radius,
+                                       current);
+      }
+    };
+    if (!lsh_any_lane(level != 0)) {
+      test_start_level(std::true_type{});
+    } else {
       lsh_keep_branch();
-      exit_distance = lsh_cell_exit_general(x_lo, y_lo, 1 << shift, px, py, dx, dy);
-      culled = lsh_bound_cannot_rise(hd0, dist0, near, two_radius_oz, inv_two_observer_radius,
-                                     current);
+      test_start_level(std::false_type{});
     }
     if (kCount) {
       ++cnt.steps;
