@@ -347,6 +347,7 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   // half the level-0 cell side in km; the level-l fallback is this 4^l (exact)
   const S fallback0 = (res * S(0.5)) * S(0.001);
   const S res_mantissa = static_cast<S>(d.resolution_mantissa);
+  const S res4 = res * S(4);  // the level-1 cell side (exact)
   const S far_step = (S(0.8) * res) * S(0.001);
   // the reference's px in [0, W-1) is floor(px) in [0, W-2]
   const S max_x = static_cast<S>(d.width - 2), max_y = static_cast<S>(d.height - 2);
@@ -377,16 +378,20 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     const S chord_h = lsh_fma(planar, lsh_fma(planar, g.c3, g.c2), g.c1);
     const S true_m = near ? s.times_1000() : lsh_fma(planar, chord_h, chord0);
     // start level: the highest l in 1..top whose cell side 4^l res fits the beam
-    // footprint, else 0. With footprint = f 2^ef and res = r 2^er (f, r in
-    // [0.5, 1)), floor(log2(footprint / res)) = ef - er - (f < r), exactly, and
-    // that l is its floor half, clamped to [0, top] (negative exactly when
-    // footprint < res). No loop, no branch, no rounding.
+    // footprint, else 0 -- 0 exactly when footprint < 4 res (top >= 1 for any
+    // DEM of at least 2 x 2). Otherwise, with footprint = f 2^ef and
+    // res = r 2^er (f, r in [0.5, 1)), floor(log2(footprint / res)) =
+    // ef - er - (f < r), exactly, and l is its floor half, clamped to top. No
+    // loop, no rounding; formed only in a wavefront where a lane needs it.
     const S footprint = true_m * static_cast<S>(kBeamWidth);
-    int footprint_exponent;
-    const S footprint_mantissa = lsh_frexp(footprint, footprint_exponent);
-    int level = (footprint_exponent - d.resolution_exponent -
-                 (footprint_mantissa < res_mantissa ? 1 : 0)) >> 1;
-    level = level < 0 ? 0 : (level < top_level ? level : top_level);
+    int level = 0;
+    auto start_level = [&]() {
+      int footprint_exponent;
+      const S footprint_mantissa = lsh_frexp(footprint, footprint_exponent);
+      const int l = (footprint_exponent - d.resolution_exponent -
+                     (footprint_mantissa < res_mantissa ? 1 : 0)) >> 1;
+      return l < 0 ? 0 : (l < top_level ? l : top_level);
+    };
 
     // -- hierarchy tests at this position (cuda_backend.py:706-868)
     // hierarchy_bilinear_bound (cuda_backend.py:47-60) -- the maximum of the
@@ -447,10 +452,11 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
                                        current);
       }
     };
-    if (!lsh_any_lane(level != 0)) {
+    if (!lsh_any_lane(footprint >= res4)) {
       test_start_level(std::true_type{});
     } else {
       lsh_keep_branch();
+      level = start_level();
       test_start_level(std::false_type{});
     }
     if (kCount) {
