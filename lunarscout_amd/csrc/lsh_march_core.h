@@ -376,14 +376,16 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     const S planar = lsh_sqrt_any(lsh_fma(plx, plx, ply * ply));
     // chord0 + c1 p + c2 p^2 + c3 p^3 (cuda_backend.py:695-698) in Horner form
     const S chord_h = lsh_fma(planar, lsh_fma(planar, g.c3, g.c2), g.c1);
-    const S true_m = near ? s.times_1000() : lsh_fma(planar, chord_h, chord0);
+    // the far-field true distance; a lane near the observer (s < 0.5 km) uses s
+    // itself, formed in the wavefronts that hold one (below)
+    S true_m = lsh_fma(planar, chord_h, chord0);
     // start level: the highest l in 1..top whose cell side 4^l res fits the beam
     // footprint, else 0 -- 0 exactly when footprint < 4 res (top >= 1 for any
     // DEM of at least 2 x 2). Otherwise, with footprint = f 2^ef and
     // res = r 2^er (f, r in [0.5, 1)), floor(log2(footprint / res)) =
     // ef - er - (f < r), exactly, and l is its floor half, clamped to top. No
     // loop, no rounding; formed only in a wavefront where a lane needs it.
-    const S footprint = true_m * static_cast<S>(kBeamWidth);
+    S footprint = true_m * static_cast<S>(kBeamWidth);
     int level = 0;
     auto start_level = [&]() {
       int footprint_exponent;
@@ -408,10 +410,11 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     // The bound test at the start level, the cell exit and the cull test. The
     // start level is 0 for every lane of a wavefront out to the distance where
     // the footprint reaches 4 res (about 55 km at 60 m/px, the rule on these
-    // DEMs): that wavefront takes a copy of the code specialised to level 0 --
-    // the cell is the level-0 cell, its bound sits at row iy, column ix of the
-    // level-0 bound copy (no level table), its side is 1 px, its fallback
-    // fallback0 -- and others the general one.
+    // DEMs): a wavefront with no lane there and none near the observer takes a
+    // copy of the code specialised to level 0 and the far field -- the cell is
+    // the level-0 cell, its bound sits at row iy, column ix of the level-0
+    // bound copy (no level table), its side is 1 px, its fallback fallback0 --
+    // and others the general one.
     int shift = 0, cx = ix, cy = iy;
     float maxh = 0.0f;
     S fallback = fallback0, exit_distance = S(0);
@@ -445,17 +448,22 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
       const V lxx0 = lsh_fma(-lz0, lz0, sq0);
       culled = lz0 * lsh_abs(lz0) <= (current * lsh_abs(current)) * lxx0;
       const bool tiny = !(lsh_abs(dx) > S(1e-8)) || !(lsh_abs(dy) > S(1e-8));
-      if (lsh_any_lane(tiny || near || !(lxx0 > V(0)))) {
+      // (no lane of a level-0 wavefront is near: the vote below sends those here)
+      if (lsh_any_lane(tiny || (!kLevel0 && near) || !(lxx0 > V(0)))) {
         lsh_keep_branch();
         exit_distance = lsh_cell_exit_general(x_lo, y_lo, side, px, py, dx, dy);
         culled = lsh_bound_cannot_rise(hd0, dist0, near, two_radius_oz, inv_two_observer_radius,
                                        current);
       }
     };
-    if (!lsh_any_lane(footprint >= res4)) {
+    if (!lsh_any_lane(near || footprint >= res4)) {
       test_start_level(std::true_type{});
     } else {
       lsh_keep_branch();
+      if (near) {
+        true_m = s.times_1000();
+        footprint = true_m * static_cast<S>(kBeamWidth);
+      }
       level = start_level();
       test_start_level(std::false_type{});
     }
