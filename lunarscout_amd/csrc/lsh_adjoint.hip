@@ -223,7 +223,8 @@ adjoint_kernel(AdjointArgs a) {
         primary.column_resolution / d.column_resolution);
     MarchView view{d.level0, d.buffer_bytes, d.bound_offset, s_off[pass], s_w[pass], s_h[pass],
                    d.width, d.height, d.level_count, d.map_resolution, d.radius, d.all_valid,
-                   d.resolution, d.resolution_exponent, d.resolution_mantissa};
+                   d.resolution, d.resolution_exponent, d.resolution_mantissa,
+                   d.box_offset, d.box_width};
     if (pass < win_pass)
       lsh_march_pass<double, double, false, true>(g, view, oz, pass, current, best, cnt);
     else

@@ -70,6 +70,8 @@ struct MarchDem {
   double resolution;        // map_resolution widened
   int32_t resolution_exponent;  // its binary exponent (frexp)
   double resolution_mantissa;   // and its mantissa in [0.5, 1)
+  uint32_t box_offset;          // element offset of the level-2 bounds from level0
+  int32_t box_width;            // level 2's width in cells; 0 when the DEM has no level 2
   int32_t level_offset[kMaxLevels];  // element offsets from level0 (level 0: 0)
   int32_t level_width[kMaxLevels];
   int32_t level_height[kMaxLevels];

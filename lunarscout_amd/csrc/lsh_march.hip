@@ -86,7 +86,8 @@ __device__ __attribute__((always_inline)) void march_pixel(
         primary.column_resolution / d.column_resolution);
     MarchView view{d.level0, d.buffer_bytes, d.bound_offset, s_off[pass], s_w[pass], s_h[pass],
                    d.width, d.height, d.level_count, d.map_resolution, d.radius, d.all_valid,
-                   d.resolution, d.resolution_exponent, d.resolution_mantissa};
+                   d.resolution, d.resolution_exponent, d.resolution_mantissa,
+                   d.box_offset, d.box_width};
     lsh_march_pass<double, V, kCap, kCount>(g, view, oz, pass, current, best, cnt);
   }
   float result = current > V(-1e29) ? static_cast<float>(current) : -INFINITY;
@@ -256,6 +257,11 @@ MarchDem march_dem(const Dem& dem) {
     m.level_width[l] = used ? dem.levels[l][2] : 1;
     m.level_height[l] = used ? dem.levels[l][3] : 1;
   }
+  // the guarded skips read the level-2 bounds (lsh_march_core.h)
+  m.box_offset = dem.level_count > 2
+                     ? static_cast<uint32_t>(m.bound_offset + m.level_offset[2])
+                     : 0u;
+  m.box_width = dem.level_count > 2 ? m.level_width[2] : 0;
   return m;
 }
 

@@ -65,6 +65,10 @@ struct MarchView {
   double resolution;
   int32_t resolution_exponent;
   double resolution_mantissa;  // res = resolution_mantissa 2^resolution_exponent
+  // the level-2 bound copy (element offset from base, row length in cells) for
+  // the guarded skips of lsh_march_pass; box_width 0 turns them off
+  uint32_t box_offset;
+  int32_t box_width;
 };
 
 // One pass's interpolated segment with its start relative to (bx, by), in the
@@ -215,6 +219,34 @@ LSH_HD bool lsh_bound_cannot_rise(V hd, V distance, bool near, V two_radius_oz,
   return near ? hd <= current * distance : far_result;  // near: distance >= 1 m
 }
 
+// Guarded skips. The cull test of a level-0 cell (above) decides on the bound B
+// of the cell's 2x2 block at the distance d = max(true_m - res, 1). Its slope is
+// tan(theta) with sin(theta) = lz / d = (hd k - d^2) / (a d), k = 2R + h + oz,
+// a = 2(R + oz), hd = B - oz: increasing in hd, and over d in [d_lo, d_hi] at
+// most (hd k - d_lo d') / (a d') with d' = d_lo for hd >= 0 and d_hi for hd < 0
+// (each term bounded on its own; the slope is not monotone in d when hd < 0).
+// The level-2 bound at cell ((ix - 7) >> 4, (iy - 7) >> 4) covers pixels
+// [16 ((ix - 7) >> 4), +32), which holds the 2x2 block of every level-0 cell
+// within 7 cells of (ix, iy). So when that bound, over the distances
+// true_m -+ 18 res, cannot reach current - 1e-9 (a margin far above the
+// rounding of either test), every level-0 cell within 7 cells of (ix, iy) whose
+// true distance lies in that range is culled (or nodata) by the method's own
+// test; and when true_m + 18 res keeps the footprint below 4 res, each such
+// position starts at level 0. lsh_guard_holds is that test for one lane.
+template <class S, class V>
+LSH_HD bool lsh_guard_holds(float box_bound, V oz, S true_m, S margin, V two_radius_oz,
+                            V two_observer_radius, V current, S res4) {
+  const V hd = static_cast<V>(box_bound) - oz;
+  const S d_hi = true_m + margin;
+  const S d_lo = lsh_max(true_m - margin, S(1));
+  const S d_sel = hd >= V(0) ? d_lo : d_hi;
+  const V num = lsh_fma(hd, two_radius_oz + hd, -(d_lo * d_sel));  // a d' sin(theta) bound
+  const V ad = two_observer_radius * d_sel;
+  const V c = current - V(1e-9);
+  return (d_hi * static_cast<S>(kBeamWidth) < res4) & (lsh_abs(num) < ad) &
+         (num * lsh_abs(num) <= (c * lsh_abs(c)) * lsh_fma(-num, num, ad * ad));
+}
+
 // a * b for 0 <= a, b < 2^24 (row index times row length: one full-rate
 // 24-bit multiply on the device instead of a quarter-rate 32-bit one)
 LSH_HD uint32_t lsh_mul24(int32_t a, int32_t b) {
@@ -356,6 +388,19 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   const S clamp_ty = (static_cast<S>(d.height) - S(1.0001)) - static_cast<S>(d.height - 2);
   const bool all_valid = d.all_valid != 0;
   const int top_level = d.level_count - 1;
+  // Guarded skips (lsh_guard_holds): after a position at which every lane of
+  // the wavefront holds the guard, the next kGuardCells positions of every lane
+  // whose cell stays within kGuardCells of the guard's cell are culls at level
+  // 0 by the method's test, so they take only the path, the cell exit and the
+  // skip. A wave-uniform count; lanes need not reach the same cells. Not with a
+  // tracer (its rows name the bound and the cull or nodata action) nor in the
+  // float32 builds (the 1e-9 margin is below their rounding).
+  constexpr bool kGuard =
+      std::is_same<Tracer, NoTrace>::value && sizeof(S) == 8 && sizeof(V) == 8;
+  constexpr int kGuardCells = 7;
+  const bool guard_on = kGuard && d.box_width > 0;
+  const S guard_margin = S(2 * (kGuardCells + 2)) * res;
+  float box_bound = 0.0f;
   RayDistance<S> s;
   s.start(lsh_max(g.s_start, S(0.001)));
   int sample_index = 0;
@@ -431,6 +476,14 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
                   : static_cast<uint32_t>(d.bound_offset + d.level_offset[level] +
                                           lsh_mul24(cy, d.level_width[level]) + cx);
       maxh = lsh_load(d, bound_index);
+      if (kLevel0 && kGuard) {
+        // the level-2 bound a guard at this position tests, read with the cell's
+        // (unconditionally, so that the cull test waits for its own load only)
+        const int gx = (ix > kGuardCells ? ix - kGuardCells : 0) >> 4;
+        const int gy = (iy > kGuardCells ? iy - kGuardCells : 0) >> 4;
+        box_bound = lsh_load(d, d.box_offset + lsh_mul24(gy, d.box_width) +
+                                    static_cast<uint32_t>(gx));
+      }
       fallback = kLevel0 ? fallback0 : lsh_ldexp(fallback0, shift);
       const S cell_m = kLevel0 ? res : lsh_ldexp(res, shift);
       // the common cell exit (formed while the bound load is in flight) and
@@ -456,7 +509,8 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
                                        current);
       }
     };
-    if (!lsh_any_lane(near || footprint >= res4)) {
+    const bool level0_wave = !lsh_any_lane(near || footprint >= res4);
+    if (level0_wave) {
       test_start_level(std::true_type{});
     } else {
       lsh_keep_branch();
@@ -556,6 +610,42 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
                 float(py + g.by), maxh, 0.f, 0.f, float(skip), nodata ? 2 : 1);
     }
     s.add(advance);
+    if constexpr (kGuard) {
+      if (guard_on && level0_wave) {
+        // a guard for the next positions (current as this position left it); its
+        // box must lie inside [0, W-2] x [0, H-2], where the march never breaks
+        const bool holds =
+            (ix >= kGuardCells) & (iy >= kGuardCells) & (ix <= d.width - 2 - kGuardCells) &
+            (iy <= d.height - 2 - kGuardCells) &
+            lsh_guard_holds(box_bound, oz, true_m, guard_margin, two_radius_oz,
+                            two_observer_radius, current, res4);
+        if (!lsh_any_lane(!holds)) {
+          // the guarded positions: the level-0 cell (floor(px), floor(py))
+          // relative to (bx, by), its exit and the skip; the run ends early (the
+          // position is then taken again as an ordinary one) when a lane's cell
+          // leaves the guard's box
+          const int guard_x = ix - g.bx, guard_y = iy - g.by;
+          for (int k = 0; k < kGuardCells; ++k) {
+            if (!s.at_most(g.s_end)) break;
+            const S gdelta = s.minus(g.s_start);
+            S gpx, gpy, gdx, gdy;
+            lsh_quartic_and_tangent(g.sx, g.a1, g.a2, g.a3, g.a4, gdelta, gpx, gdx);
+            lsh_quartic_and_tangent(g.sy, g.b1, g.b2, g.b3, g.b4, gdelta, gpy, gdy);
+            const int rx = static_cast<int>(lsh_floor(gpx)), ry = static_cast<int>(lsh_floor(gpy));
+            const bool inside =
+                (static_cast<unsigned>(rx - guard_x + kGuardCells) <= 2u * kGuardCells) &
+                (static_cast<unsigned>(ry - guard_y + kGuardCells) <= 2u * kGuardCells);
+            if (lsh_any_lane(!inside)) break;
+            const S gexit = lsh_cell_exit(rx, ry, 1, gpx, gpy, gdx, gdy);
+            if (kCount) {
+              ++cnt.steps;
+              ++cnt.tests;
+            }
+            s.add((gexit > S(0) ? gexit : fallback0) + S(0.0001));
+          }
+        }
+      }
+    }
   }
 }
 

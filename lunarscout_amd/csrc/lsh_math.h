@@ -22,6 +22,16 @@ LSH_MHD bool lsh_any_lane(bool p) {
 #endif
 }
 
+// A wave-uniform integer as the compiler's scalar value (read from the first
+// active lane), so that branches on it are scalar branches; x on the host.
+LSH_MHD int lsh_uniform(int x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_readfirstlane(x);
+#else
+  return x;
+#endif
+}
+
 // An empty statement the compiler may not move or remove: placed in the rare
 // side of a wave-uniform branch, it keeps the branch a branch (the compiler
 // would otherwise evaluate both sides for every lane and select).

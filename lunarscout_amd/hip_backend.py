@@ -294,7 +294,7 @@ class HipSession(_lib.Session):
         self._output = self.march_device(dems, self._segments, tile_column=tile_column,
                                          tile_row=tile_row,
                                          observer_elevation_m=observer_elevation_m,
-                                         flags=flags, output=self._output)
+                                         flags=flags, output=self._output, counters=counter_buf)
         values = self._output.to_host()
         if counter_buf is not None:
             c = counter_buf.to_host()

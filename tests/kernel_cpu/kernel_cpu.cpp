@@ -63,8 +63,8 @@ namespace {
 template <class S, class V, class Tracer>
 float ray(const kc_pyramid* pyrs, int D, const float* segments, int C, int az, int column,
           int row, int tile_column, int tile_row, int tile_width, int subpatch, float obs_h,
-          bool cap, const PassTables& t, int32_t* best_out, lsh::MarchCounters& cnt,
-          const Tracer& tr) {
+          bool cap, bool guarded, const PassTables& t, int32_t* best_out,
+          lsh::MarchCounters& cnt, const Tracer& tr) {
   const kc_dem& primary = pyrs[0].dem;
   lsh::PixelSelection sel = lsh::lsh_select(column, row, tile_column, tile_row, tile_width,
                                             subpatch, primary.width, primary.height);
@@ -84,7 +84,11 @@ float ray(const kc_pyramid* pyrs, int D, const float* segments, int C, int az, i
                      t.lw[p], t.lh[p], pyrs[p].dem.width, pyrs[p].dem.height,
                      pyrs[p].level_count, t.mapres[p], float(pyrs[p].dem.projection[0]),
                      t.all_valid[p], double(t.mapres[p]), res_exponent(t.mapres[p]),
-                     res_mantissa(t.mapres[p])};
+                     res_mantissa(t.mapres[p]), 0u, 0};
+    if (guarded && pyrs[p].level_count > 2) {  // the level-2 bounds, as march_dem sets them
+      v.box_offset = uint32_t(t.bound_offset[p] + t.off[p][2]);
+      v.box_width = t.lw[p][2];
+    }
     if (cap)
       lsh::lsh_march_pass<S, V, true, true>(g, v, oz, p, current, best, cnt, tr);
     else
@@ -149,17 +153,17 @@ float ray_mode(int mode, const kc_pyramid* pyrs, int D, const float* segments, i
                int column, int row, int tile_column, int tile_row, int tile_width, int subpatch,
                float obs_h, const PassTables& t, int32_t* best, lsh::MarchCounters& cnt,
                const Tracer& tr) {
-  bool cap = mode & 1;
+  bool cap = mode & 1, guarded = !(mode & 8);
   switch (mode & 6) {
     case 0:
       return ray<float, float>(pyrs, D, segments, C, az, column, row, tile_column, tile_row,
-                               tile_width, subpatch, obs_h, cap, t, best, cnt, tr);
+                               tile_width, subpatch, obs_h, cap, guarded, t, best, cnt, tr);
     case 2:
       return ray<double, float>(pyrs, D, segments, C, az, column, row, tile_column, tile_row,
-                                tile_width, subpatch, obs_h, cap, t, best, cnt, tr);
+                                tile_width, subpatch, obs_h, cap, guarded, t, best, cnt, tr);
     default:
       return ray<double, double>(pyrs, D, segments, C, az, column, row, tile_column, tile_row,
-                                 tile_width, subpatch, obs_h, cap, t, best, cnt, tr);
+                                 tile_width, subpatch, obs_h, cap, guarded, t, best, cnt, tr);
   }
 }
 
@@ -167,7 +171,7 @@ float ray_mode(int mode, const kc_pyramid* pyrs, int D, const float* segments, i
 
 extern "C" {
 
-// flags: 1 = cap, 2 = float64 schedule, 4 = float64 values.
+// flags: 1 = cap, 2 = float64 schedule, 4 = float64 values, 8 = no guarded skips.
 // counters: int64[3] (samples, tests, steps).
 int kcpu_horizon_tile(const kc_pyramid* pyrs, int32_t D, const float* segments, int32_t N,
                       int32_t tile_column, int32_t tile_row, int32_t tile_width,
@@ -261,7 +265,7 @@ int kcpu_march_segment(const float* segment, const kc_pyramid* pyr, double obser
                    t.off[0], t.lw[0], t.lh[0], pyr->dem.width, pyr->dem.height,
                    pyr->level_count, t.mapres[0], float(pyr->dem.projection[0]),
                    t.all_valid[0], double(t.mapres[0]), res_exponent(t.mapres[0]),
-                   res_mantissa(t.mapres[0])};
+                   res_mantissa(t.mapres[0]), 0u, 0};
   lsh::MarchCounters cnt{0, 0, 0};
   int n = 0;
   RowTrace tr{{}, rows, cap, &n};

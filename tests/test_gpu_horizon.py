@@ -178,6 +178,18 @@ def test_counters_match_the_cpu_build():
     assert got[3] >= got[2]
 
 
+def test_horizon_tile_returns_the_march_counters():
+    dems = [synthetic.fbm_crater_dem(512, 20.0, 3)]
+    s = session()
+    _, got = s.horizon_tile(dems, tile_column=128, tile_row=128, azimuth_count=64, counters=True)
+    segs = s.build_subpatch_segments(dems, tile_column=128, tile_row=128, azimuth_count=64)
+    _, _, expected = kernel_cpu.horizon_tile([oracle.OraclePyramid(d) for d in dems], segs,
+                                             tile_column=128, tile_row=128, mode=6)
+    assert expected[0] > 0
+    np.testing.assert_array_equal(got[:3], expected)
+    assert got[3] >= got[2]
+
+
 def test_captured_production_buffer_with_the_diagnostic_cap():
     """One pixel x 1440 azimuths x 2 passes of the captured C# buffer (cap on).
 

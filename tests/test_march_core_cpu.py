@@ -114,3 +114,25 @@ def test_edge_rays_on_rough_terrain_match_the_oracle():
         ref, ref_rows, got, got_rows = _both(seg, pyramid, observer_z=float(elevation[100, 64]))
         _assert_same_trace(ref_rows, got_rows)
         assert got == pytest.approx(ref, rel=1e-9, abs=1e-12)
+
+
+def test_guarded_skips_leave_every_output_and_counter_unchanged():
+    """The march with guarded skips (a level-2 bound test that proves the next
+    level-0 cells culled, then only their exits) against the same build with
+    them off: slopes, winning samples and the traversal counters equal bit for
+    bit on a two-DEM site (512^2 @5 m + 1024^2 @60 m), interior and edge pixels."""
+    from lunarscout_amd import synthetic
+
+    dems = synthetic.site_dems(synthetic.FULL_SITE, scale=4)
+    pyramids = [oracle.OraclePyramid(d) for d in dems]
+    for column, row in [(128, 128), (0, 384)]:
+        segs = oracle.segments(dems, tile_column=column, tile_row=row, tile_width=128,
+                               azimuth_count=1440)
+        pixels = np.arange(0, 16384, 331, dtype=np.int32)
+        kw = dict(tile_column=column, tile_row=row, pixels=pixels)
+        on, won, con = kernel_cpu.horizon_tile(pyramids, segs, mode=6, **kw)
+        off, woff, coff = kernel_cpu.horizon_tile(pyramids, segs, mode=6 | 8, **kw)
+        assert con[0] > 0
+        np.testing.assert_array_equal(on.view(np.uint32), off.view(np.uint32))
+        np.testing.assert_array_equal(won, woff)
+        np.testing.assert_array_equal(con, coff)
