@@ -404,6 +404,10 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   RayDistance<S> s;
   s.start(lsh_max(g.s_start, S(0.001)));
   int sample_index = 0;
+  // after a guard fails, the next kGuardBackoff level-0 positions do not test
+  // one (a failing region tends to keep failing)
+  constexpr int kGuardBackoff = 8;
+  int cooldown = 0;
   while (s.at_most(g.s_end)) {
     // -- position (cuda_backend.py:672-705)
     const S delta = s.minus(g.s_start);
@@ -611,7 +615,9 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     }
     s.add(advance);
     if constexpr (kGuard) {
-      if (guard_on && level0_wave) {
+      if (guard_on && level0_wave && cooldown > 0) {
+        --cooldown;
+      } else if (guard_on && level0_wave) {
         // a guard for the next positions (current as this position left it); its
         // box must lie inside [0, W-2] x [0, H-2], where the march never breaks
         const bool holds =
@@ -619,7 +625,9 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
             (iy <= d.height - 2 - kGuardCells) &
             lsh_guard_holds(box_bound, oz, true_m, guard_margin, two_radius_oz,
                             two_observer_radius, current, res4);
-        if (!lsh_any_lane(!holds)) {
+        if (lsh_any_lane(!holds)) {
+          cooldown = kGuardBackoff;
+        } else {
           // the guarded positions: the level-0 cell (floor(px), floor(py))
           // relative to (bx, by), its exit and the skip; the run ends early (the
           // position is then taken again as an ordinary one) when a lane's cell
