@@ -235,7 +235,7 @@ LSH_HD bool lsh_bound_cannot_rise(V hd, V distance, bool near, V two_radius_oz,
 // position starts at level 0. lsh_guard_holds is that test for one lane.
 template <class S, class V>
 LSH_HD bool lsh_guard_holds(float box_bound, V oz, S true_m, S margin, V two_radius_oz,
-                            V two_observer_radius, V current, S res4) {
+                            V two_observer_radius, V current, S footprint_lo, S footprint_hi) {
   const V hd = static_cast<V>(box_bound) - oz;
   const S d_hi = true_m + margin;
   const S d_lo = lsh_max(true_m - margin, S(1));
@@ -243,7 +243,8 @@ LSH_HD bool lsh_guard_holds(float box_bound, V oz, S true_m, S margin, V two_rad
   const V num = lsh_fma(hd, two_radius_oz + hd, -(d_lo * d_sel));  // a d' sin(theta) bound
   const V ad = two_observer_radius * d_sel;
   const V c = current - V(1e-9);
-  return (d_hi * static_cast<S>(kBeamWidth) < res4) & (lsh_abs(num) < ad) &
+  return (d_lo * static_cast<S>(kBeamWidth) >= footprint_lo) &
+         (d_hi * static_cast<S>(kBeamWidth) < footprint_hi) & (lsh_abs(num) < ad) &
          (num * lsh_abs(num) <= (c * lsh_abs(c)) * lsh_fma(-num, num, ad * ad));
 }
 
@@ -399,7 +400,6 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
       std::is_same<Tracer, NoTrace>::value && sizeof(S) == 8 && sizeof(V) == 8;
   constexpr int kGuardCells = 7;
   const bool guard_on = kGuard && d.box_width > 0;
-  const S guard_margin = S(2 * (kGuardCells + 2)) * res;
   float box_bound = 0.0f;
   RayDistance<S> s;
   s.start(lsh_max(g.s_start, S(0.001)));
@@ -480,13 +480,21 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
                   : static_cast<uint32_t>(d.bound_offset + d.level_offset[level] +
                                           lsh_mul24(cy, d.level_width[level]) + cx);
       maxh = lsh_load(d, bound_index);
-      if (kLevel0 && kGuard) {
-        // the level-2 bound a guard at this position tests, read with the cell's
-        // (unconditionally, so that the cull test waits for its own load only)
-        const int gx = (ix > kGuardCells ? ix - kGuardCells : 0) >> 4;
-        const int gy = (iy > kGuardCells ? iy - kGuardCells : 0) >> 4;
-        box_bound = lsh_load(d, d.box_offset + lsh_mul24(gy, d.box_width) +
-                                    static_cast<uint32_t>(gx));
+      if (kGuard) {
+        // the level-(l+2) bound a guard at this position tests, read with the
+        // cell's (unconditionally, so that the cull test waits for its own load
+        // only; a level without it reads an unused in-range value)
+        const int gx = (cx > kGuardCells ? cx - kGuardCells : 0) >> 4;
+        const int gy = (cy > kGuardCells ? cy - kGuardCells : 0) >> 4;
+        if (kLevel0) {
+          box_bound = lsh_load(d, d.box_offset + lsh_mul24(gy, d.box_width) +
+                                      static_cast<uint32_t>(gx));
+        } else {
+          const int box_level = level + 2 <= top_level ? level + 2 : top_level;
+          box_bound = lsh_load(d, static_cast<uint32_t>(d.bound_offset + d.level_offset[box_level]) +
+                                      lsh_mul24(gy, d.level_width[box_level]) +
+                                      static_cast<uint32_t>(gx));
+        }
       }
       fallback = kLevel0 ? fallback0 : lsh_ldexp(fallback0, shift);
       const S cell_m = kLevel0 ? res : lsh_ldexp(res, shift);
@@ -525,6 +533,7 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
       level = start_level();
       test_start_level(std::false_type{});
     }
+    const int start = level;  // (a descent below lowers level)
     if (kCount) {
       ++cnt.steps;
       if (nodata || culled || level != 0) ++cnt.tests;
@@ -615,41 +624,52 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     }
     s.add(advance);
     if constexpr (kGuard) {
-      if (guard_on && level0_wave && cooldown > 0) {
+      // a guard for the next positions when every lane starts at the same level
+      // l (none near the observer) and the DEM has level l + 2 (current as this
+      // position left it); its box must lie inside [0, W-2] x [0, H-2], where
+      // the march never breaks
+      const int gl = level0_wave ? 0 : lsh_uniform(start);
+      const bool candidate =
+          guard_on && (level0_wave || (gl + 2 <= top_level && !lsh_any_lane(start != gl || near)));
+      if (candidate && cooldown > 0) {
         --cooldown;
-      } else if (guard_on && level0_wave) {
-        // a guard for the next positions (current as this position left it); its
-        // box must lie inside [0, W-2] x [0, H-2], where the march never breaks
+      } else if (candidate) {
+        const int gsh = 2 * gl;
+        const int gcx = ix >> gsh, gcy = iy >> gsh;
+        const S side_m = lsh_ldexp(res, gsh);
         const bool holds =
-            (ix >= kGuardCells) & (iy >= kGuardCells) & (ix <= d.width - 2 - kGuardCells) &
-            (iy <= d.height - 2 - kGuardCells) &
-            lsh_guard_holds(box_bound, oz, true_m, guard_margin, two_radius_oz,
-                            two_observer_radius, current, res4);
+            (gcx >= kGuardCells) & (gcy >= kGuardCells) &
+            (((gcx + kGuardCells + 1) << gsh) <= d.width - 1) &
+            (((gcy + kGuardCells + 1) << gsh) <= d.height - 1) &
+            lsh_guard_holds(box_bound, oz, true_m, S(2 * (kGuardCells + 2)) * side_m,
+                            two_radius_oz, two_observer_radius, current,
+                            gl > 0 ? side_m : S(0), side_m * S(4));
         if (lsh_any_lane(!holds)) {
           cooldown = kGuardBackoff;
         } else {
-          // the guarded positions: the level-0 cell (floor(px), floor(py))
-          // relative to (bx, by), its exit and the skip; the run ends early (the
-          // position is then taken again as an ordinary one) when a lane's cell
-          // leaves the guard's box
-          const int guard_x = ix - g.bx, guard_y = iy - g.by;
+          // the guarded positions: the level-l cell of the position, its exit
+          // and the skip; the run ends early (the position is then taken again
+          // as an ordinary one) when a lane's cell leaves the guard's box
+          const S fallback_l = lsh_ldexp(fallback0, gsh);
           for (int k = 0; k < kGuardCells; ++k) {
             if (!s.at_most(g.s_end)) break;
             const S gdelta = s.minus(g.s_start);
             S gpx, gpy, gdx, gdy;
             lsh_quartic_and_tangent(g.sx, g.a1, g.a2, g.a3, g.a4, gdelta, gpx, gdx);
             lsh_quartic_and_tangent(g.sy, g.b1, g.b2, g.b3, g.b4, gdelta, gpy, gdy);
-            const int rx = static_cast<int>(lsh_floor(gpx)), ry = static_cast<int>(lsh_floor(gpy));
+            const int kx = (static_cast<int>(lsh_floor(gpx)) + g.bx) >> gsh;
+            const int ky = (static_cast<int>(lsh_floor(gpy)) + g.by) >> gsh;
             const bool inside =
-                (static_cast<unsigned>(rx - guard_x + kGuardCells) <= 2u * kGuardCells) &
-                (static_cast<unsigned>(ry - guard_y + kGuardCells) <= 2u * kGuardCells);
+                (static_cast<unsigned>(kx - gcx + kGuardCells) <= 2u * kGuardCells) &
+                (static_cast<unsigned>(ky - gcy + kGuardCells) <= 2u * kGuardCells);
             if (lsh_any_lane(!inside)) break;
-            const S gexit = lsh_cell_exit(rx, ry, 1, gpx, gpy, gdx, gdy);
+            const S gexit = lsh_cell_exit((kx << gsh) - g.bx, (ky << gsh) - g.by, 1 << gsh, gpx,
+                                          gpy, gdx, gdy);
             if (kCount) {
               ++cnt.steps;
               ++cnt.tests;
             }
-            s.add((gexit > S(0) ? gexit : fallback0) + S(0.0001));
+            s.add((gexit > S(0) ? gexit : fallback_l) + S(0.0001));
           }
         }
       }

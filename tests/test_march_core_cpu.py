@@ -120,15 +120,16 @@ def test_guarded_skips_leave_every_output_and_counter_unchanged():
     """The march with guarded skips (a level-2 bound test that proves the next
     level-0 cells culled, then only their exits) against the same build with
     them off: slopes, winning samples and the traversal counters equal bit for
-    bit on a two-DEM site (512^2 @5 m + 1024^2 @60 m), interior and edge pixels."""
+    bit on a two-DEM site (1024^2 @5 m + 2048^2 @60 m: the regional rays reach
+    level 1 beyond 55 km), interior and edge pixels."""
     from lunarscout_amd import synthetic
 
-    dems = synthetic.site_dems(synthetic.FULL_SITE, scale=4)
+    dems = synthetic.site_dems(synthetic.FULL_SITE, scale=2)
     pyramids = [oracle.OraclePyramid(d) for d in dems]
-    for column, row in [(128, 128), (0, 384)]:
+    for column, row in [(384, 384), (0, 896)]:
         segs = oracle.segments(dems, tile_column=column, tile_row=row, tile_width=128,
                                azimuth_count=1440)
-        pixels = np.arange(0, 16384, 331, dtype=np.int32)
+        pixels = np.arange(0, 16384, 683, dtype=np.int32)
         kw = dict(tile_column=column, tile_row=row, pixels=pixels)
         on, won, con = kernel_cpu.horizon_tile(pyramids, segs, mode=6, **kw)
         off, woff, coff = kernel_cpu.horizon_tile(pyramids, segs, mode=6 | 8, **kw)
