@@ -219,20 +219,22 @@ LSH_HD bool lsh_bound_cannot_rise(V hd, V distance, bool near, V two_radius_oz,
   return near ? hd <= current * distance : far_result;  // near: distance >= 1 m
 }
 
-// Guarded skips. The cull test of a level-0 cell (above) decides on the bound B
-// of the cell's 2x2 block at the distance d = max(true_m - res, 1). Its slope is
-// tan(theta) with sin(theta) = lz / d = (hd k - d^2) / (a d), k = 2R + h + oz,
-// a = 2(R + oz), hd = B - oz: increasing in hd, and over d in [d_lo, d_hi] at
-// most (hd k - d_lo d') / (a d') with d' = d_lo for hd >= 0 and d_hi for hd < 0
-// (each term bounded on its own; the slope is not monotone in d when hd < 0).
-// The level-2 bound at cell ((ix - 7) >> 4, (iy - 7) >> 4) covers pixels
-// [16 ((ix - 7) >> 4), +32), which holds the 2x2 block of every level-0 cell
-// within 7 cells of (ix, iy). So when that bound, over the distances
-// true_m -+ 18 res, cannot reach current - 1e-9 (a margin far above the
-// rounding of either test), every level-0 cell within 7 cells of (ix, iy) whose
-// true distance lies in that range is culled (or nodata) by the method's own
-// test; and when true_m + 18 res keeps the footprint below 4 res, each such
-// position starts at level 0. lsh_guard_holds is that test for one lane.
+// Guarded skips. The cull test of a level-l cell (below) decides on the bound B
+// of the cell's 2x2 block at the distance d = max(true_m - side, 1), side =
+// 4^l res. Its slope is tan(theta) with sin(theta) = lz / d =
+// (hd k - d^2) / (a d), k = 2R + h + oz, a = 2(R + oz), hd = B - oz:
+// increasing in hd, and over d in [d_lo, d_hi] at most (hd k - d_lo d') / (a d')
+// with d' = d_lo for hd >= 0 and d_hi for hd < 0 (each term bounded on its own;
+// the slope is not monotone in d when hd < 0). A window of 16 x 16 level-l cells
+// starting at (wx, wy) lies with the 2x2 blocks of its cells inside the
+// 32 x 32 cells [16 (wx >> 4), +32) x [16 (wy >> 4), +32) whose maximum is the
+// level-(l+2) bound at (wx >> 4, wy >> 4) (the mips are maxima). So when that
+// bound, over the distances true_m -+ 32 side (more than any cell of the window
+// lies from the position), cannot reach current - 1e-9 (a margin far above the
+// rounding of either test), every position of the lane whose level-l cell lies
+// in the window is culled (or nodata) by the method's own test; and when that
+// distance range keeps the footprint in [4^l res, 4^(l+1) res), each such
+// position starts at level l. lsh_guard_holds is that test for one lane.
 template <class S, class V>
 LSH_HD bool lsh_guard_holds(float box_bound, V oz, S true_m, S margin, V two_radius_oz,
                             V two_observer_radius, V current, S footprint_lo, S footprint_hi) {
@@ -390,15 +392,21 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   const bool all_valid = d.all_valid != 0;
   const int top_level = d.level_count - 1;
   // Guarded skips (lsh_guard_holds): after a position at which every lane of
-  // the wavefront holds the guard, the next kGuardCells positions of every lane
-  // whose cell stays within kGuardCells of the guard's cell are culls at level
-  // 0 by the method's test, so they take only the path, the cell exit and the
-  // skip. A wave-uniform count; lanes need not reach the same cells. Not with a
-  // tracer (its rows name the bound and the cull or nodata action) nor in the
-  // float32 builds (the 1e-9 margin is below their rounding).
+  // the wavefront starts at the same level and holds the guard, the next
+  // kGuardSteps positions of every lane whose cell stays in its window are culls
+  // at that level by the method's test, so they take only the path, the cell
+  // exit and the skip; the window runs from one cell behind the lane's cell to
+  // 14 ahead in the direction the lane moves. A wave-uniform count; lanes need
+  // not reach the same cells. Not with a tracer (its rows name the bound and the
+  // cull or nodata action) nor in the float32 builds (the 1e-9 margin is below
+  // their rounding).
   constexpr bool kGuard =
       std::is_same<Tracer, NoTrace>::value && sizeof(S) == 8 && sizeof(V) == 8;
-  constexpr int kGuardCells = 7;
+  constexpr int kGuardWindow = 16, kGuardSteps = 14;
+  // a lane's window starts one cell behind its cell in the direction it moves
+  auto window_start = [](int c, S tangent) {
+    return tangent > S(0) ? c - 1 : c - (kGuardWindow - 2);
+  };
   const bool guard_on = kGuard && d.box_width > 0;
   float box_bound = 0.0f;
   RayDistance<S> s;
@@ -484,8 +492,8 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
         // the level-(l+2) bound a guard at this position tests, read with the
         // cell's (unconditionally, so that the cull test waits for its own load
         // only; a level without it reads an unused in-range value)
-        const int gx = (cx > kGuardCells ? cx - kGuardCells : 0) >> 4;
-        const int gy = (cy > kGuardCells ? cy - kGuardCells : 0) >> 4;
+        const int wx = window_start(cx, dx), wy = window_start(cy, dy);
+        const int gx = (wx > 0 ? wx : 0) >> 4, gy = (wy > 0 ? wy : 0) >> 4;
         if (kLevel0) {
           box_bound = lsh_load(d, d.box_offset + lsh_mul24(gy, d.box_width) +
                                       static_cast<uint32_t>(gx));
@@ -635,13 +643,12 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
         --cooldown;
       } else if (candidate) {
         const int gsh = 2 * gl;
-        const int gcx = ix >> gsh, gcy = iy >> gsh;
+        const int wx = window_start(ix >> gsh, dx), wy = window_start(iy >> gsh, dy);
         const S side_m = lsh_ldexp(res, gsh);
         const bool holds =
-            (gcx >= kGuardCells) & (gcy >= kGuardCells) &
-            (((gcx + kGuardCells + 1) << gsh) <= d.width - 1) &
-            (((gcy + kGuardCells + 1) << gsh) <= d.height - 1) &
-            lsh_guard_holds(box_bound, oz, true_m, S(2 * (kGuardCells + 2)) * side_m,
+            (wx >= 0) & (wy >= 0) & (((wx + kGuardWindow) << gsh) <= d.width - 1) &
+            (((wy + kGuardWindow) << gsh) <= d.height - 1) &
+            lsh_guard_holds(box_bound, oz, true_m, S(2 * kGuardWindow) * side_m,
                             two_radius_oz, two_observer_radius, current,
                             gl > 0 ? side_m : S(0), side_m * S(4));
         if (lsh_any_lane(!holds)) {
@@ -651,7 +658,7 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
           // and the skip; the run ends early (the position is then taken again
           // as an ordinary one) when a lane's cell leaves the guard's box
           const S fallback_l = lsh_ldexp(fallback0, gsh);
-          for (int k = 0; k < kGuardCells; ++k) {
+          for (int k = 0; k < kGuardSteps; ++k) {
             if (!s.at_most(g.s_end)) break;
             const S gdelta = s.minus(g.s_start);
             S gpx, gpy, gdx, gdy;
@@ -659,9 +666,8 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
             lsh_quartic_and_tangent(g.sy, g.b1, g.b2, g.b3, g.b4, gdelta, gpy, gdy);
             const int kx = (static_cast<int>(lsh_floor(gpx)) + g.bx) >> gsh;
             const int ky = (static_cast<int>(lsh_floor(gpy)) + g.by) >> gsh;
-            const bool inside =
-                (static_cast<unsigned>(kx - gcx + kGuardCells) <= 2u * kGuardCells) &
-                (static_cast<unsigned>(ky - gcy + kGuardCells) <= 2u * kGuardCells);
+            const bool inside = (static_cast<unsigned>(kx - wx) < kGuardWindow) &
+                                (static_cast<unsigned>(ky - wy) < kGuardWindow);
             if (lsh_any_lane(!inside)) break;
             const S gexit = lsh_cell_exit((kx << gsh) - g.bx, (ky << gsh) - g.by, 1 << gsh, gpx,
                                           gpy, gdx, gdy);
