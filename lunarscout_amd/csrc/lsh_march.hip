@@ -75,14 +75,26 @@ __device__ __attribute__((always_inline)) void march_pixel(
   V current = V(-1e30);
   int32_t best = -1;
   MarchCounters cnt{0, 0, 0};
-  const int64_t seg_stride = static_cast<int64_t>(D) * kSegmentFields;
+  const int seg_stride = D * kSegmentFields;
   const float* seg_az = segments + static_cast<int64_t>(az) * C * seg_stride;
   for (int pass = 0; pass < D; ++pass) {
     const MarchDem& d = dems[pass];
     const float* p = seg_az + pass * kSegmentFields;
+    // the selection as opaque copies: what is derived from it is formed again
+    // in each pass (a few instructions) instead of being held across the march,
+    // where registers are short (held values spill)
+    PixelSelection ps = sel;
+    lsh_keep_value(ps.i00);
+    lsh_keep_value(ps.i10);
+    lsh_keep_value(ps.i01);
+    lsh_keep_value(ps.i11);
+    lsh_keep_value(ps.cl);
+    lsh_keep_value(ps.cr);
+    lsh_keep_value(ps.ct);
+    lsh_keep_value(ps.cb);
     const PassSegment<double> g = lsh_interpolate<double>(
-        p + sel.i00 * seg_stride, p + sel.i10 * seg_stride, p + sel.i01 * seg_stride,
-        p + sel.i11 * seg_stride, sel, column, row,
+        p + ps.i00 * seg_stride, p + ps.i10 * seg_stride, p + ps.i01 * seg_stride,
+        p + ps.i11 * seg_stride, ps, column, row,
         primary.column_resolution / d.column_resolution);
     MarchView view{d.level0, d.buffer_bytes, d.bound_offset, s_off[pass], s_w[pass], s_h[pass],
                    d.width, d.height, d.level_count, d.map_resolution, d.radius, d.all_valid,
