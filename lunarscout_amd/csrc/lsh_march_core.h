@@ -409,6 +409,15 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   };
   const bool guard_on = kGuard && d.box_width > 0;
   float box_bound = 0.0f;
+  // the true distance in metres of a position away from the observer
+  // (cuda_backend.py:690-698): the planar distance from the segment start and
+  // chord0 + c1 p + c2 p^2 + c3 p^3 in Horner form
+  auto far_true_m = [&](S px, S py) {
+    const S plx = (px - g.sx) * res, ply = (py - g.sy) * res;
+    const S planar = lsh_sqrt_any(lsh_fma(plx, plx, ply * ply));
+    const S chord_h = lsh_fma(planar, lsh_fma(planar, g.c3, g.c2), g.c1);
+    return lsh_fma(planar, chord_h, chord0);
+  };
   RayDistance<S> s;
   s.start(lsh_max(g.s_start, S(0.001)));
   int sample_index = 0;
@@ -429,13 +438,9 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
     if (!(ax >= S(0) && ay >= S(0) && ax <= max_x && ay <= max_y)) break;
     const int ix = static_cast<int>(ax), iy = static_cast<int>(ay);
     const bool near = s.below(S(0.5));
-    const S plx = (px - g.sx) * res, ply = (py - g.sy) * res;
-    const S planar = lsh_sqrt_any(lsh_fma(plx, plx, ply * ply));
-    // chord0 + c1 p + c2 p^2 + c3 p^3 (cuda_backend.py:695-698) in Horner form
-    const S chord_h = lsh_fma(planar, lsh_fma(planar, g.c3, g.c2), g.c1);
     // the far-field true distance; a lane near the observer (s < 0.5 km) uses s
     // itself, formed in the wavefronts that hold one (below)
-    S true_m = lsh_fma(planar, chord_h, chord0);
+    S true_m = far_true_m(px, py);
     // start level: the highest l in 1..top whose cell side 4^l res fits the beam
     // footprint, else 0 -- 0 exactly when footprint < 4 res (top >= 1 for any
     // DEM of at least 2 x 2). Otherwise, with footprint = f 2^ef and
@@ -655,27 +660,75 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
           cooldown = kGuardBackoff;
         } else {
           // the guarded positions: the level-l cell of the position, its exit
-          // and the skip; the run ends early (the position is then taken again
-          // as an ordinary one) when a lane's cell leaves the guard's box
+          // and the skip. When a run ends with every lane still in its window,
+          // the guard is tested again at the next position (which the new
+          // window then covers too) instead of an ordinary position; a run
+          // ends early (the position is then taken again as an ordinary one)
+          // when a lane's cell leaves its window, and the guarded stretch ends
+          // when a new guard does not hold
           const S fallback_l = lsh_ldexp(fallback0, gsh);
-          for (int k = 0; k < kGuardSteps; ++k) {
-            if (!s.at_most(g.s_end)) break;
+          const S margin = S(2 * kGuardWindow) * side_m;
+          int gwx = wx, gwy = wy;
+          for (;;) {
+            bool left_window = false;
+            for (int k = 0; k < kGuardSteps; ++k) {
+              if (!s.at_most(g.s_end)) break;
+              const S gdelta = s.minus(g.s_start);
+              S gpx, gpy, gdx, gdy;
+              lsh_quartic_and_tangent(g.sx, g.a1, g.a2, g.a3, g.a4, gdelta, gpx, gdx);
+              lsh_quartic_and_tangent(g.sy, g.b1, g.b2, g.b3, g.b4, gdelta, gpy, gdy);
+              const int kx = (static_cast<int>(lsh_floor(gpx)) + g.bx) >> gsh;
+              const int ky = (static_cast<int>(lsh_floor(gpy)) + g.by) >> gsh;
+              const bool inside = (static_cast<unsigned>(kx - gwx) < kGuardWindow) &
+                                  (static_cast<unsigned>(ky - gwy) < kGuardWindow);
+              if (lsh_any_lane(!inside)) {
+                left_window = true;
+                break;
+              }
+              const S gexit = lsh_cell_exit((kx << gsh) - g.bx, (ky << gsh) - g.by, 1 << gsh,
+                                            gpx, gpy, gdx, gdy);
+              if (kCount) {
+                ++cnt.steps;
+                ++cnt.tests;
+              }
+              s.add((gexit > S(0) ? gexit : fallback_l) + S(0.0001));
+            }
+            if (lsh_any_lane(left_window)) break;
+            const bool active = s.at_most(g.s_end);
+            if (!lsh_any_lane(active)) break;
+            // the guard again, at the next position of the lanes still marching
+            // (none is near the observer: s only grows)
             const S gdelta = s.minus(g.s_start);
             S gpx, gpy, gdx, gdy;
             lsh_quartic_and_tangent(g.sx, g.a1, g.a2, g.a3, g.a4, gdelta, gpx, gdx);
             lsh_quartic_and_tangent(g.sy, g.b1, g.b2, g.b3, g.b4, gdelta, gpy, gdy);
             const int kx = (static_cast<int>(lsh_floor(gpx)) + g.bx) >> gsh;
             const int ky = (static_cast<int>(lsh_floor(gpy)) + g.by) >> gsh;
-            const bool inside = (static_cast<unsigned>(kx - wx) < kGuardWindow) &
-                                (static_cast<unsigned>(ky - wy) < kGuardWindow);
-            if (lsh_any_lane(!inside)) break;
-            const S gexit = lsh_cell_exit((kx << gsh) - g.bx, (ky << gsh) - g.by, 1 << gsh, gpx,
-                                          gpy, gdx, gdy);
-            if (kCount) {
-              ++cnt.steps;
-              ++cnt.tests;
+            const int nwx = window_start(kx, gdx), nwy = window_start(ky, gdy);
+            const S g_true_m = far_true_m(gpx, gpy);
+            const int box_level = gl + 2;
+            const float g_box = lsh_load(
+                d, static_cast<uint32_t>(d.bound_offset + d.level_offset[box_level]) +
+                       lsh_mul24((nwy > 0 ? nwy : 0) >> 4, d.level_width[box_level]) +
+                       static_cast<uint32_t>((nwx > 0 ? nwx : 0) >> 4));
+            const bool again =
+                !active |
+                ((nwx >= 0) & (nwy >= 0) & (((nwx + kGuardWindow) << gsh) <= d.width - 1) &
+                 (((nwy + kGuardWindow) << gsh) <= d.height - 1) &
+                 lsh_guard_holds(g_box, oz, g_true_m, margin, two_radius_oz, two_observer_radius,
+                                 current, gl > 0 ? side_m : S(0), side_m * S(4)));
+            if (lsh_any_lane(!again)) break;
+            if (active) {
+              const S gexit = lsh_cell_exit((kx << gsh) - g.bx, (ky << gsh) - g.by, 1 << gsh, gpx,
+                                            gpy, gdx, gdy);
+              if (kCount) {
+                ++cnt.steps;
+                ++cnt.tests;
+              }
+              s.add((gexit > S(0) ? gexit : fallback_l) + S(0.0001));
             }
-            s.add((gexit > S(0) ? gexit : fallback_l) + S(0.0001));
+            gwx = nwx;
+            gwy = nwy;
           }
         }
       }
