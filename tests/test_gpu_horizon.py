@@ -111,6 +111,40 @@ def test_march_equals_cpu_build_of_the_kernel_bit_for_bit():
     assert same.all()
 
 
+def test_march_with_nodata_equals_cpu_build_and_oracle():
+    """Two DEMs (1024^2 @5 m + 2048^2 @60 m) with nodata in both -- scattered NaN
+    pixels, a -32768 block, an infinite pixel -- so rays take nodata skips,
+    samples with invalid taps and guarded runs over nodata cells: the GPU equals
+    the CPU build of the kernel bit for bit and the float64 oracle within the
+    parity bar."""
+    rng = np.random.default_rng(17)
+    dems = []
+    for d in synthetic.site_dems(synthetic.FULL_SITE, scale=2):
+        z = np.array(d.elevation_m, dtype=np.float32)
+        h, w = z.shape
+        z[rng.random(z.shape) < 0.003] = np.nan
+        z[h // 5:h // 5 + 120, w // 2:w // 2 + 140] = -32768.0
+        z[h // 3, w // 4] = np.inf
+        dems.append(DemGrid(z, d.geo_transform, d.projection))
+    tile = (384, 384)
+    segs = oracle.segments(dems, tile_column=tile[0], tile_row=tile[1], tile_width=128,
+                           azimuth_count=1440)
+    pyrs, px = _tile_case(dems, tile, pixels=192, seed=3)
+    s = session()
+    from lunarscout_amd._lib import DeviceBuffer
+
+    seg_buf = DeviceBuffer.from_host(s, segs)
+    out = s.march_device(dems, seg_buf, tile_column=tile[0], tile_row=tile[1]).to_host()
+    cpu, _, _ = kernel_cpu.horizon_tile(pyrs, segs, tile_column=tile[0], tile_row=tile[1],
+                                        pixels=px, mode=6)
+    same = (out[px] == cpu) | (np.isneginf(out[px]) & np.isneginf(cpu))
+    print("nodata site: bit-identical fraction", same.mean(), flush=True)
+    assert same.all()
+    ref, _ = oracle.horizon_tile(pyrs, segs, tile_column=tile[0], tile_row=tile[1], pixels=px,
+                                 float64=True)
+    assert_horizon_parity(out[px], ref, "nodata site tile")
+
+
 def test_config1_tile_parity_against_float64_oracle():
     """1024x1024 px at 20 m, one DEM, central tile (512, 512), 1440 azimuths."""
     dems = [synthetic.fbm_crater_dem(1024, 20.0, 0)]

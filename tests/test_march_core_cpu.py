@@ -137,3 +137,31 @@ def test_guarded_skips_leave_every_output_and_counter_unchanged():
         np.testing.assert_array_equal(on.view(np.uint32), off.view(np.uint32))
         np.testing.assert_array_equal(won, woff)
         np.testing.assert_array_equal(con, coff)
+
+
+def test_guarded_skips_with_nodata_holes_leave_every_output_unchanged():
+    """As above on the two-DEM site with nodata in both DEMs: scattered NaN pixels,
+    a -32768 block and an infinite pixel, so guard windows hold nodata cells and
+    whole nodata bounds."""
+    from lunarscout_amd import synthetic
+
+    rng = np.random.default_rng(17)
+    dems = []
+    for d in synthetic.site_dems(synthetic.FULL_SITE, scale=2):
+        z = np.array(d.elevation_m, dtype=np.float32)
+        h, w = z.shape
+        z[rng.random(z.shape) < 0.003] = np.nan
+        z[h // 5:h // 5 + 120, w // 2:w // 2 + 140] = -32768.0
+        z[h // 3, w // 4] = np.inf
+        dems.append(DemGrid(z, d.geo_transform, d.projection))
+    pyramids = [oracle.OraclePyramid(d) for d in dems]
+    segs = oracle.segments(dems, tile_column=384, tile_row=384, tile_width=128,
+                           azimuth_count=1440)
+    pixels = np.arange(0, 16384, 683, dtype=np.int32)
+    kw = dict(tile_column=384, tile_row=384, pixels=pixels)
+    on, won, con = kernel_cpu.horizon_tile(pyramids, segs, mode=6, **kw)
+    off, woff, coff = kernel_cpu.horizon_tile(pyramids, segs, mode=6 | 8, **kw)
+    assert con[0] > 0
+    np.testing.assert_array_equal(on.view(np.uint32), off.view(np.uint32))
+    np.testing.assert_array_equal(won, woff)
+    np.testing.assert_array_equal(con, coff)
