@@ -712,7 +712,7 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
                        lsh_mul24((nwy > 0 ? nwy : 0) >> 4, d.level_width[box_level]) +
                        static_cast<uint32_t>((nwx > 0 ? nwx : 0) >> 4));
             const bool again =
-                !active |
+                (!active) |
                 ((nwx >= 0) & (nwy >= 0) & (((nwx + kGuardWindow) << gsh) <= d.width - 1) &
                  (((nwy + kGuardWindow) << gsh) <= d.height - 1) &
                  lsh_guard_holds(g_box, oz, g_true_m, margin, two_radius_oz, two_observer_radius,
