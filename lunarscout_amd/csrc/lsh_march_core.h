@@ -421,8 +421,8 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   RayDistance<S> s;
   s.start(lsh_max(g.s_start, S(0.001)));
   int sample_index = 0;
-  // after a guard fails, the next kGuardBackoff level-0 positions do not test
-  // one (a failing region tends to keep failing)
+  // after a guard fails, the next kGuardBackoff positions that could test one do
+  // not (a failing region tends to keep failing)
   constexpr int kGuardBackoff = 8;
   int cooldown = 0;
   while (s.at_most(g.s_end)) {
