@@ -407,6 +407,16 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
   auto window_start = [](int c, S tangent) {
     return tangent > S(0) ? c - 1 : c - (kGuardWindow - 2);
   };
+  // The guard's distance range takes a position of the window (within
+  // 15 sqrt(2) cells of the lane's position) to be at most 1.4 x that far in
+  // true distance from the lane's: the chord polynomial's slope
+  // |c1 + 2 c2 p + 3 c3 p^2| stays below 1.4 for planar distances p up to twice
+  // the pass's path length (1.0 within a fraction of a percent in the polar
+  // stereographic frame); a lane whose fit does not is never guarded.
+  const S chord_reach = g.s_end * S(2000);
+  const bool chord_tame = lsh_abs(g.c1) + lsh_abs(S(2) * g.c2) * chord_reach +
+                              lsh_abs(S(3) * g.c3) * (chord_reach * chord_reach) <=
+                          S(1.4);
   const bool guard_on = kGuard && d.box_width > 0;
   float box_bound = 0.0f;
   // the true distance in metres of a position away from the observer
@@ -651,7 +661,7 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
         const int wx = window_start(ix >> gsh, dx), wy = window_start(iy >> gsh, dy);
         const S side_m = lsh_ldexp(res, gsh);
         const bool holds =
-            (wx >= 0) & (wy >= 0) & (((wx + kGuardWindow) << gsh) <= d.width - 1) &
+            chord_tame & (wx >= 0) & (wy >= 0) & (((wx + kGuardWindow) << gsh) <= d.width - 1) &
             (((wy + kGuardWindow) << gsh) <= d.height - 1) &
             lsh_guard_holds(box_bound, oz, true_m, S(2 * kGuardWindow) * side_m,
                             two_radius_oz, two_observer_radius, current,
@@ -713,7 +723,8 @@ LSH_HD void lsh_march_pass(const PassSegment<S>& g, const MarchView& d, V oz, in
                        static_cast<uint32_t>((nwx > 0 ? nwx : 0) >> 4));
             const bool again =
                 (!active) |
-                ((nwx >= 0) & (nwy >= 0) & (((nwx + kGuardWindow) << gsh) <= d.width - 1) &
+                (chord_tame & (nwx >= 0) & (nwy >= 0) &
+                 (((nwx + kGuardWindow) << gsh) <= d.width - 1) &
                  (((nwy + kGuardWindow) << gsh) <= d.height - 1) &
                  lsh_guard_holds(g_box, oz, g_true_m, margin, two_radius_oz, two_observer_radius,
                                  current, gl > 0 ? side_m : S(0), side_m * S(4)));
