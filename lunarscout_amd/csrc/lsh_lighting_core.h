@@ -74,18 +74,57 @@ LSH_LHD Frame pixel_frame(const LightArgs& a, int row, int column) {
   return f;
 }
 
-// _azimuth_elevation_deg (psr.py:50-58) in float64: az = atan2(e, n) in degrees
-// wrapped into [0, 360) as Python's % does (atan2 lies in (-180, 180], so one
-// addition of 360 to a negative angle is that remainder), el = atan2(u, |(e, n)|).
-LSH_LHD void azimuth_elevation(const Frame& f, const double* v, double& az, double& el) {
-  double e[3];
+// _azimuth_elevation_deg (psr.py:50-58) in float64: the body vector in the
+// pixel's (east, north, up) frame and its horizontal length, then az = atan2(e, n)
+// in degrees wrapped into [0, 360) as Python's % does (atan2 lies in
+// (-180, 180], so one addition of 360 to a negative angle is that remainder),
+// el = atan2(u, |(e, n)|).
+LSH_LHD void body_vector(const Frame& f, const double* v, double* e, double& horizontal) {
   for (int j = 0; j < 3; ++j)
     e[j] = v[0] * f.rot[0][j] + v[1] * f.rot[1][j] + v[2] * f.rot[2][j] + f.tr[j];
+  horizontal = lsh_sqrt_any(e[0] * e[0] + e[1] * e[1]);
+}
+LSH_LHD void azimuth_elevation_of(const double* e, double horizontal, double& az, double& el) {
   double a = atan2_finite(e[0], e[1]) * (180.0 / kPi);
   if (a < 0.0) a += 360.0;
   az = a;
-  const double horizontal = lsh_sqrt_any(e[0] * e[0] + e[1] * e[1]);
   el = atan2_finite(e[2], horizontal) * (180.0 / kPi);
+}
+LSH_LHD void azimuth_elevation(const Frame& f, const double* v, double& az, double& el) {
+  double e[3], horizontal;
+  body_vector(f, v, e, horizontal);
+  azimuth_elevation_of(e, horizontal, az, el);
+}
+
+// Whole-row decisions of the sun fraction. With lo and hi the lowest and
+// highest bins of a row (every bin finite) and c the tallest slice half-height,
+// a disk whose centre elevation is below lo - c - 2e-3 deg has every slice top
+// below every interpolated horizon of the row, so the walk adds nothing (0); one
+// above hi + c + 2e-3 deg has every slice wholly above, so the walk adds every
+// full slice (max_photons exactly, 1). The margin is far above the float32
+// rounding of the walk. Both tests are taken on the body vector before any
+// arctangent: el < t <=> u < |(e, n)| tan(t) for |t| < 90 deg. A row with a
+// non-finite bin, or a bound beyond +-89 deg, decides nothing.
+struct RowDecision {
+  double tan_dark, tan_lit;
+};
+template <class Row>
+LSH_LHD RowDecision row_decision(const Row& hz, int N, const LightArgs& a) {
+  float lo = 1e30f, hi = -1e30f;
+  bool finite = true;
+  for (int b = 0; b < N; ++b) {
+    const float v = hz[b];
+    finite = finite && v > -1e30f && v < 1e30f;
+    lo = v < lo ? v : lo;
+    hi = v > hi ? v : hi;
+  }
+  float c = 0.0f;
+  for (int i = 0; i < 16; ++i) c = a.half_circle[i] > c ? a.half_circle[i] : c;
+  const double dark = static_cast<double>(lo) - c - 2e-3, lit = static_cast<double>(hi) + c + 2e-3;
+  RowDecision r;
+  r.tan_dark = finite && dark > -89.0 ? tan(dark * (kPi / 180.0)) : -INFINITY;
+  r.tan_lit = finite && lit < 89.0 ? tan(lit * (kPi / 180.0)) : INFINITY;
+  return r;
 }
 
 template <bool kAzimuthMajor>
@@ -234,10 +273,20 @@ LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origi
 
   if (sun && a.sun_count > 0 && (a.sun_bytes || a.sun_fraction || a.sun_mean)) {
     float acc = 0.0f;
+    const RowDecision rd = row_decision(hz, N, a);
     for (int t = 0; t < a.sun_count; ++t) {
-      double az, el;
-      azimuth_elevation(f, a.sun + 3 * t, az, el);
-      float fr = sun_fraction(hz, N, static_cast<float>(az), static_cast<float>(el), a);
+      double e[3], horizontal;
+      body_vector(f, a.sun + 3 * t, e, horizontal);
+      float fr;
+      if (e[2] < horizontal * rd.tan_dark) {
+        fr = 0.0f;
+      } else if (e[2] > horizontal * rd.tan_lit) {
+        fr = 1.0f;
+      } else {
+        double az, el;
+        azimuth_elevation_of(e, horizontal, az, el);
+        fr = sun_fraction(hz, N, static_cast<float>(az), static_cast<float>(el), a);
+      }
       acc += fr;
       int64_t o = (static_cast<int64_t>(tile) * a.sun_count + t) * kPatchPixels + pixel;
       if (a.sun_bytes) a.sun_bytes[o] = static_cast<uint8_t>(255.0f * fr);
