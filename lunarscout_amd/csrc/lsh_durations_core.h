@@ -59,10 +59,17 @@ struct PixelSignals {
   const LightArgs& a;
   Frame f;
   HorizonRow<kAzimuthMajor> hz;
+  RowDecision rd;  // whole-row decisions of the sun fraction (row_decision)
   // sun fraction (BuilderSunFraction) or centre margin el - lerp(horizon), float32
   LSH_LHD float sun(int t, int signal) const {
+    double e[3], horizontal;
+    body_vector(f, a.sun + 3 * t, e, horizontal);
+    if (signal == LSH_SIGNAL_SUN_FRACTION) {
+      if (e[2] < horizontal * rd.tan_dark) return 0.0f;
+      if (e[2] > horizontal * rd.tan_lit) return 1.0f;
+    }
     double az, el;
-    azimuth_elevation(f, a.sun + 3 * t, az, el);
+    azimuth_elevation_of(e, horizontal, az, el);
     if (signal == LSH_SIGNAL_SUN_FRACTION)
       return sun_fraction(hz, a.N, static_cast<float>(az), static_cast<float>(el), a);
     return static_cast<float>(el - horizon_at(hz, a.N, az));
@@ -126,10 +133,11 @@ LSH_LHD void duration_pixel(const DurationArgs& a, int tile, int pixel, lsh_tile
     for (int k = 0; k < bands; ++k) a.out[k * stride + slot] = NAN;
     return;
   }
-  PixelSignals<kAzimuthMajor> sig{la, pixel_frame(la, row, column), {}};
+  PixelSignals<kAzimuthMajor> sig{la, pixel_frame(la, row, column), {}, {-INFINITY, INFINITY}};
   sig.hz.N = la.N;
   sig.hz.base = kAzimuthMajor ? la.horizons + static_cast<int64_t>(tile) * kPatchPixels * la.N + pixel
                               : la.horizons + slot * la.N;
+  if (a.sun_signal == LSH_SIGNAL_SUN_FRACTION) sig.rd = row_decision(sig.hz, la.N, la);
 
   if (a.reducer == LSH_REDUCE_LONGEST_DURATION) {
     for (int k = 0; k < a.K; ++k) a.longest[k * stride + slot] = 0.0;
