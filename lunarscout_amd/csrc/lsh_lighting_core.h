@@ -105,26 +105,45 @@ LSH_LHD void azimuth_elevation(const Frame& f, const double* v, double& az, doub
 // rounding of the walk. Both tests are taken on the body vector before any
 // arctangent: el < t <=> u < |(e, n)| tan(t) for |t| < 90 deg. A row with a
 // non-finite bin, or a bound beyond +-89 deg, decides nothing.
+struct RowExtremes {
+  float lo, hi;  // lowest and highest bin
+  bool finite;   // every bin finite
+};
+template <class Row>
+LSH_LHD RowExtremes row_extremes(const Row& hz, int N) {
+  RowExtremes x{1e30f, -1e30f, true};
+  for (int b = 0; b < N; ++b) {
+    const float v = hz[b];
+    x.finite = x.finite && v > -1e30f && v < 1e30f;
+    x.lo = v < x.lo ? v : x.lo;
+    x.hi = v > x.hi ? v : x.hi;
+  }
+  return x;
+}
+// tan of an elevation bound in degrees, or the infinity that decides nothing
+LSH_LHD double tan_bound(bool usable, double deg, double none) {
+  return usable && deg > -89.0 && deg < 89.0 ? tan(deg * (kPi / 180.0)) : none;
+}
 struct RowDecision {
   double tan_dark, tan_lit;
 };
-template <class Row>
-LSH_LHD RowDecision row_decision(const Row& hz, int N, const LightArgs& a) {
-  float lo = 1e30f, hi = -1e30f;
-  bool finite = true;
-  for (int b = 0; b < N; ++b) {
-    const float v = hz[b];
-    finite = finite && v > -1e30f && v < 1e30f;
-    lo = v < lo ? v : lo;
-    hi = v > hi ? v : hi;
-  }
+LSH_LHD RowDecision row_decision(const RowExtremes& x, const LightArgs& a) {
   float c = 0.0f;
   for (int i = 0; i < 16; ++i) c = a.half_circle[i] > c ? a.half_circle[i] : c;
-  const double dark = static_cast<double>(lo) - c - 2e-3, lit = static_cast<double>(hi) + c + 2e-3;
-  RowDecision r;
-  r.tan_dark = finite && dark > -89.0 ? tan(dark * (kPi / 180.0)) : -INFINITY;
-  r.tan_lit = finite && lit < 89.0 ? tan(lit * (kPi / 180.0)) : INFINITY;
-  return r;
+  return RowDecision{tan_bound(x.finite, static_cast<double>(x.lo) - c - 2e-3, -INFINITY),
+                     tan_bound(x.finite, static_cast<double>(x.hi) + c + 2e-3, INFINITY)};
+}
+template <class Row>
+LSH_LHD RowDecision row_decision(const Row& hz, int N, const LightArgs& a) {
+  return row_decision(row_extremes(hz, N), a);
+}
+// The same for the sign of a margin el - lerp(horizon) against a threshold:
+// below lo + threshold - 1e-6 deg it is below the threshold wherever the body
+// stands, above hi + threshold + 1e-6 deg above it (the margin's float64
+// rounding is ~1e-13 deg).
+LSH_LHD RowDecision margin_decision(const RowExtremes& x, double threshold) {
+  return RowDecision{tan_bound(x.finite, static_cast<double>(x.lo) + threshold - 1e-6, -INFINITY),
+                     tan_bound(x.finite, static_cast<double>(x.hi) + threshold + 1e-6, INFINITY)};
 }
 
 template <bool kAzimuthMajor>
@@ -296,9 +315,20 @@ LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origi
   }
   if (earth && a.earth_count > 0 && (a.earth_margin || a.earth_visible)) {
     int visible = 0;
+    // the visible count alone is decided for the whole row where the Earth's
+    // centre clears the row's extremes; the margins need every value
+    const RowDecision rd = a.earth_margin ? RowDecision{-INFINITY, INFINITY}
+                                          : margin_decision(row_extremes(hz, N), 0.0);
     for (int t = 0; t < a.earth_count; ++t) {
+      double e[3], horizontal;
+      body_vector(f, a.earth + 3 * t, e, horizontal);
+      if (e[2] < horizontal * rd.tan_dark) continue;
+      if (e[2] > horizontal * rd.tan_lit) {
+        ++visible;
+        continue;
+      }
       double az, el;
-      azimuth_elevation(f, a.earth + 3 * t, az, el);
+      azimuth_elevation_of(e, horizontal, az, el);
       double m = el - horizon_at(hz, N, az);
       visible += m >= 0.0;
       if (a.earth_margin)
@@ -312,10 +342,21 @@ LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origi
     // the mask alone is decided by the first vector that lights the limb
     // (psr.py:136-139); the margin needs every vector
     const bool mask_only = a.psr_margin == nullptr;
+    // with the mask alone, a vector below the row's lowest bin (less the
+    // threshold) cannot light the limb and one above its highest does
+    const RowDecision rd = mask_only ? margin_decision(row_extremes(hz, N), a.psr_threshold)
+                                     : RowDecision{-INFINITY, INFINITY};
     double best = -1e300;
     for (int t = 0; t < a.psr_count; ++t) {
+      double e[3], horizontal;
+      body_vector(f, a.psr_vectors + 3 * t, e, horizontal);
+      if (e[2] < horizontal * rd.tan_dark) continue;
+      if (e[2] > horizontal * rd.tan_lit) {
+        best = INFINITY;
+        break;
+      }
       double az, el;
-      azimuth_elevation(f, a.psr_vectors + 3 * t, az, el);
+      azimuth_elevation_of(e, horizontal, az, el);
       double m = el - horizon_at(hz, N, az);
       best = m > best ? m : best;
       if (mask_only && best > a.psr_threshold) break;

@@ -223,3 +223,31 @@ def test_disk_walk_stays_in_its_row_for_any_bin_count(bins):
         fractions = np.stack(list(cpu.iter_patch_fraction_tiles(dem, rows, vectors, tile_y=0,
                                                                 tile_x=0)))
         np.testing.assert_array_equal(fractions[:, y, x], np.ones(az.size, np.float32))
+
+
+def test_row_decisions_leave_fractions_visibility_and_mask_unchanged():
+    """The visible fraction and the PSR mask asked for alone (whole-row decisions
+    on) equal those derived with the Earth margins and the PSR margin asked for
+    too (decisions off), and the sun fractions equal the oracle's float32 walk."""
+    dem = synthetic.fbm_crater_dem(256, 5.0, 1)
+    rng = np.random.default_rng(2)
+    az = np.radians(np.arange(1440) * 0.25)
+    hz = (rng.uniform(-2.5, 4.0, (128, 128, 1)) + 1.2 * np.sin(3 * az)[None, None, :]).astype(np.float32)
+    sun = synthetic.sun_vectors(dem, 96, 4.0, seed=8)
+    earth = synthetic.earth_vectors(96, 4.0, seed=9)
+    reduced, _ = oracle.psr_reduce(dem, sun)
+    rows = hz.reshape(1, 16384, 1440)
+    common = dict(sun=sun, earth=earth, psr_vectors=reduced)
+    alone = CpuLightingSession(threads=4).lighting_tiles(
+        dem, [(0, 128)], rows, sun_fraction=True, earth_visible=True, psr=True, **common)
+    full = CpuLightingSession(threads=4).lighting_tiles(
+        dem, [(0, 128)], rows, sun_fraction=True, earth_margin=True, earth_visible=True, psr=True,
+        psr_margin=True, **common)
+    np.testing.assert_array_equal(alone["earth_visible"], full["earth_visible"])
+    np.testing.assert_array_equal(alone["psr"], full["psr"])
+    ref = oracle.lighting_patch(dem, hz, sun, tile_y=128, tile_x=0, float32_slices=True)
+    fraction = alone["sun_fraction"][0].reshape(-1, 128, 128)
+    np.testing.assert_array_equal(fraction, ref["fraction"].astype(np.float32))
+    decided = np.mean((fraction == 0) | (fraction == 1))
+    assert 0.2 < decided < 1.0, decided  # the case exercises both decided and walked times
+    assert 0 < int((alone["psr"] == 255).sum()) < 16384
