@@ -34,10 +34,13 @@ struct SeriesList {
 template <bool kAzimuthMajor>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LSH_LIGHT_WAVES_PER_EU)))
 lighting_kernel(LightArgs a, SeriesList list) {
+  // each thread's sector tables, [sector][thread] (consecutive lanes, consecutive banks)
+  __shared__ float s_dark[kSectors][256], s_lit[kSectors][256];
   const int tile = blockIdx.y;
   const int pixel = blockIdx.x * blockDim.x + threadIdx.x;
   if (pixel >= kPatchPixels) return;
-  light_pixel<kAzimuthMajor>(a, tile, pixel, a.tiles[tile], list.series[blockIdx.z]);
+  const SectorTable sectors{&s_dark[0][threadIdx.x], &s_lit[0][threadIdx.x], 256};
+  light_pixel<kAzimuthMajor>(a, tile, pixel, a.tiles[tile], sectors, list.series[blockIdx.z]);
 }
 
 }  // namespace
@@ -70,12 +73,14 @@ void run_lighting_host(const LightArgs& a, int threads) {
   n = n < 1 ? 1 : (n > 256 ? 256 : n);
   auto body = [&a, items, n](int t) {
     const int64_t begin = items * t / n, end = items * (t + 1) / n;
+    float dark[kSectors], lit[kSectors];
+    const SectorTable sectors{dark, lit, 1};
     for (int64_t i = begin; i < end; ++i) {
       const int tile = static_cast<int>(i / kPatchPixels), pixel = static_cast<int>(i % kPatchPixels);
       if (a.azimuth_major)
-        light_pixel<true>(a, tile, pixel, a.tiles[tile]);
+        light_pixel<true>(a, tile, pixel, a.tiles[tile], sectors);
       else
-        light_pixel<false>(a, tile, pixel, a.tiles[tile]);
+        light_pixel<false>(a, tile, pixel, a.tiles[tile], sectors);
     }
   };
   std::vector<std::thread> pool;

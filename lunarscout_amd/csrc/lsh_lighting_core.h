@@ -120,6 +120,22 @@ LSH_LHD RowExtremes row_extremes(const Row& hz, int N) {
   }
   return x;
 }
+// The disk walk also reads past the row's extremes: a disk centre within
+// sun_half_angle + half a bin of azimuth 0 puts the walk's first position left
+// of bin 0 (a negative fraction), and its first slices extrapolate from the
+// pairs (bin 0, bin 1) or (bin N-1, bin 0) back to at most 2 h[0] - h[1] or
+// 2 h[N-1] - h[0]. Those two values widen the extremes the disk tests use for
+// the bins at azimuth 0.
+template <class Row>
+LSH_LHD void add_walk_extrapolation(const Row& hz, int N, RowExtremes& x) {
+  const float h0 = hz[0], h1 = hz[1], hn = hz[N - 1];
+  const float a = h0 + (h0 - h1), b = hn + (hn - h0);
+  x.finite = x.finite && a > -1e30f && a < 1e30f && b > -1e30f && b < 1e30f;
+  x.lo = a < x.lo ? a : x.lo;
+  x.lo = b < x.lo ? b : x.lo;
+  x.hi = a > x.hi ? a : x.hi;
+  x.hi = b > x.hi ? b : x.hi;
+}
 // tan of an elevation bound in degrees, or the infinity that decides nothing
 LSH_LHD double tan_bound(bool usable, double deg, double none) {
   return usable && deg > -89.0 && deg < 89.0 ? tan(deg * (kPi / 180.0)) : none;
@@ -135,7 +151,9 @@ LSH_LHD RowDecision row_decision(const RowExtremes& x, const LightArgs& a) {
 }
 template <class Row>
 LSH_LHD RowDecision row_decision(const Row& hz, int N, const LightArgs& a) {
-  return row_decision(row_extremes(hz, N), a);
+  RowExtremes x = row_extremes(hz, N);
+  add_walk_extrapolation(hz, N, x);
+  return row_decision(x, a);
 }
 // The same for the sign of a margin el - lerp(horizon) against a threshold:
 // below lo + threshold - 1e-6 deg it is below the threshold wherever the body
@@ -144,6 +162,66 @@ LSH_LHD RowDecision row_decision(const Row& hz, int N, const LightArgs& a) {
 LSH_LHD RowDecision margin_decision(const RowExtremes& x, double threshold) {
   return RowDecision{tan_bound(x.finite, static_cast<double>(x.lo) + threshold - 1e-6, -INFINITY),
                      tan_bound(x.finite, static_cast<double>(x.hi) + threshold + 1e-6, INFINITY)};
+}
+
+// Sector decisions: the same tests on the extremes of the bins of one of 16
+// azimuth sectors of 22.5 deg, widened by 4 bins (1 deg) on either side -- more
+// than the disk walk reaches beyond the centre's azimuth (-0.4 .. +0.86 deg) or
+// the horizon lerp (one bin), and than a body on a sector edge can be assigned
+// to the neighbouring sector by rounding. The sector comes from the signs and
+// ratio of the body's east and north components (no arctangent). The tangents
+// are kept as float32 (the tables live in LDS); their rounding moves a bound
+// by at most 2e-6 deg, inside the margins (2e-3 deg for the disk, 1e-4 deg
+// for a margin's sign). Rows of another bin count than 1440 keep the
+// whole-row decisions in every sector.
+constexpr int kSectors = 16;
+struct SectorTable {
+  float* dark;  // tan of the sector's "below" bound, element j at dark[j * stride]
+  float* lit;
+  int stride;
+};
+LSH_LHD int sector_of(double east, double north) {
+  constexpr double kTan225 = 0.41421356237309504880, kTan675 = 2.41421356237309504880;
+  const double ae = east < 0.0 ? -east : east, an = north < 0.0 ? -north : north;
+  const int quadrant = east >= 0.0 ? (north >= 0.0 ? 0 : 1) : (north < 0.0 ? 2 : 3);
+  int sub = (ae >= kTan225 * an) + (ae >= an) + (ae >= kTan675 * an);  // angle from the N-S axis
+  if (quadrant == 1 || quadrant == 3) sub = 3 - sub;
+  return 4 * quadrant + sub;
+}
+// below: deg offset of the "below" bound from a sector's lowest bin (negative),
+// above: of the "above" bound from its highest bin
+template <class Row>
+LSH_LHD void fill_sectors(const Row& hz, int N, double below, double above, bool disk_walk,
+                          SectorTable t) {
+  if (N != 1440) {
+    RowExtremes x = row_extremes(hz, N);
+    if (disk_walk) add_walk_extrapolation(hz, N, x);
+    const float d = static_cast<float>(tan_bound(x.finite, static_cast<double>(x.lo) + below, -INFINITY));
+    const float l = static_cast<float>(tan_bound(x.finite, static_cast<double>(x.hi) + above, INFINITY));
+    for (int j = 0; j < kSectors; ++j) {
+      t.dark[j * t.stride] = d;
+      t.lit[j * t.stride] = l;
+    }
+    return;
+  }
+  constexpr int kWidth = 1440 / kSectors, kExtra = 4;
+  for (int j = 0; j < kSectors; ++j) {
+    RowExtremes x{1e30f, -1e30f, true};
+    for (int k = -kExtra; k < kWidth + kExtra; ++k) {
+      int b = j * kWidth + k;
+      b = b < 0 ? b + 1440 : (b >= 1440 ? b - 1440 : b);
+      const float v = hz[b];
+      x.finite = x.finite && v > -1e30f && v < 1e30f;
+      x.lo = v < x.lo ? v : x.lo;
+      x.hi = v > x.hi ? v : x.hi;
+    }
+    // the sectors on either side of azimuth 0 (a centre there may fall in either)
+    if (disk_walk && (j == 0 || j == kSectors - 1)) add_walk_extrapolation(hz, N, x);
+    t.dark[j * t.stride] =
+        static_cast<float>(tan_bound(x.finite, static_cast<double>(x.lo) + below, -INFINITY));
+    t.lit[j * t.stride] =
+        static_cast<float>(tan_bound(x.finite, static_cast<double>(x.hi) + above, INFINITY));
+  }
 }
 
 template <bool kAzimuthMajor>
@@ -259,7 +337,7 @@ constexpr unsigned kAllSeries = kSunSeries | kEarthSeries | kPsrSeries;
 // series walked in order.
 template <bool kAzimuthMajor>
 LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origin,
-                         unsigned series = kAllSeries) {
+                         SectorTable sectors, unsigned series = kAllSeries) {
   const int line = pixel / 128, sample = pixel % 128;
   const int row = origin.tile_row + line, column = origin.tile_column + sample;
   const int64_t tile_pixels = static_cast<int64_t>(tile) * kPatchPixels;
@@ -292,14 +370,18 @@ LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origi
 
   if (sun && a.sun_count > 0 && (a.sun_bytes || a.sun_fraction || a.sun_mean)) {
     float acc = 0.0f;
-    const RowDecision rd = row_decision(hz, N, a);
+    float c = 0.0f;  // the tallest slice half-height
+    for (int i = 0; i < 16; ++i) c = a.half_circle[i] > c ? a.half_circle[i] : c;
+    fill_sectors(hz, N, -static_cast<double>(c) - 2e-3, static_cast<double>(c) + 2e-3, true,
+                 sectors);
     for (int t = 0; t < a.sun_count; ++t) {
       double e[3], horizontal;
       body_vector(f, a.sun + 3 * t, e, horizontal);
+      const int j = sector_of(e[0], e[1]) * sectors.stride;
       float fr;
-      if (e[2] < horizontal * rd.tan_dark) {
+      if (e[2] < horizontal * static_cast<double>(sectors.dark[j])) {
         fr = 0.0f;
-      } else if (e[2] > horizontal * rd.tan_lit) {
+      } else if (e[2] > horizontal * static_cast<double>(sectors.lit[j])) {
         fr = 1.0f;
       } else {
         double az, el;
@@ -317,15 +399,18 @@ LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origi
     int visible = 0;
     // the visible count alone is decided for the whole row where the Earth's
     // centre clears the row's extremes; the margins need every value
-    const RowDecision rd = a.earth_margin ? RowDecision{-INFINITY, INFINITY}
-                                          : margin_decision(row_extremes(hz, N), 0.0);
+    const bool decide = a.earth_margin == nullptr;
+    if (decide) fill_sectors(hz, N, -1e-4, 1e-4, false, sectors);
     for (int t = 0; t < a.earth_count; ++t) {
       double e[3], horizontal;
       body_vector(f, a.earth + 3 * t, e, horizontal);
-      if (e[2] < horizontal * rd.tan_dark) continue;
-      if (e[2] > horizontal * rd.tan_lit) {
-        ++visible;
-        continue;
+      if (decide) {
+        const int j = sector_of(e[0], e[1]) * sectors.stride;
+        if (e[2] < horizontal * static_cast<double>(sectors.dark[j])) continue;
+        if (e[2] > horizontal * static_cast<double>(sectors.lit[j])) {
+          ++visible;
+          continue;
+        }
       }
       double az, el;
       azimuth_elevation_of(e, horizontal, az, el);

@@ -251,3 +251,29 @@ def test_row_decisions_leave_fractions_visibility_and_mask_unchanged():
     decided = np.mean((fraction == 0) | (fraction == 1))
     assert 0.2 < decided < 1.0, decided  # the case exercises both decided and walked times
     assert 0 < int((alone["psr"] == 255).sum()) < 16384
+
+
+def test_row_decisions_see_the_walk_extrapolate_left_of_bin_zero():
+    """A disk centred within half a bin plus the disk radius of azimuth 0 starts
+    its walk left of bin 0 and extrapolates from bins (0, 1): a row whose lowest
+    bin is 1 deg can still show the disk an extrapolated 0 deg horizon, and one
+    whose highest is 5 deg can hide it behind an extrapolated 6 deg. The
+    fractions (decided or walked) equal the oracle's float32 walk."""
+    from lunarscout_amd.geometry import local_direction_vector
+
+    dem = synthetic.fbm_crater_dem(256, 5.0, 1)
+    row, col = 128 + 40, 40
+    cases = [((1.0, 2.0, 1.0), 0.3), ((4.0, 2.0, 5.0), 5.5)]  # (h[0], h[1], h[1439]), elevation
+    for (h0, h1, hn), el in cases:
+        rowv = np.full(1440, 5.0 if el < 3 else 3.0, np.float32)
+        rowv[0], rowv[1], rowv[1439] = h0, h1, hn
+        hz = np.broadcast_to(rowv, (128, 128, 1440)).astype(np.float32)
+        azs = [0.02, 0.1, 0.2, 0.3, 0.38, 0.5, 359.9]
+        vec = np.stack([local_direction_vector(dem, row, col, a, el, 1.496e11) for a in azs])
+        out = CpuLightingSession(threads=2).lighting_tiles(
+            dem, [(0, 128)], hz.reshape(1, 16384, 1440), sun=vec, sun_fraction=True)
+        got = out["sun_fraction"][0].reshape(-1, 128, 128)
+        ref = oracle.lighting_patch(dem, hz, vec, tile_y=128, tile_x=0, float32_slices=True)
+        np.testing.assert_array_equal(got, ref["fraction"].astype(np.float32))
+        partial = got[:, 40, 40]
+        assert np.any((partial > 0) & (partial < 1)), partial  # the extrapolated cases
