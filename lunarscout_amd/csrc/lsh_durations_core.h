@@ -137,7 +137,8 @@ LSH_LHD void duration_pixel(const DurationArgs& a, int tile, int pixel, lsh_tile
   sig.hz.N = la.N;
   sig.hz.base = kAzimuthMajor ? la.horizons + static_cast<int64_t>(tile) * kPatchPixels * la.N + pixel
                               : la.horizons + slot * la.N;
-  if (a.sun_signal == LSH_SIGNAL_SUN_FRACTION) sig.rd = row_decision(sig.hz, la.N, la);
+  if (a.sun_signal == LSH_SIGNAL_SUN_FRACTION && a.T >= kDecideMinTimes)
+    sig.rd = row_decision(sig.hz, la.N, la);
 
   if (a.reducer == LSH_REDUCE_LONGEST_DURATION) {
     for (int k = 0; k < a.K; ++k) a.longest[k * stride + slot] = 0.0;

@@ -175,6 +175,9 @@ LSH_LHD RowDecision margin_decision(const RowExtremes& x, double threshold) {
 // for a margin's sign). Rows of another bin count than 1440 keep the
 // whole-row decisions in every sector.
 constexpr int kSectors = 16;
+// A series decides only when it is long enough to repay reading the whole row
+// first (the tables cost about one pass over the row's bins)
+constexpr int kDecideMinTimes = 512;
 struct SectorTable {
   float* dark;  // tan of the sector's "below" bound, element j at dark[j * stride]
   float* lit;
@@ -372,16 +375,18 @@ LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origi
     float acc = 0.0f;
     float c = 0.0f;  // the tallest slice half-height
     for (int i = 0; i < 16; ++i) c = a.half_circle[i] > c ? a.half_circle[i] : c;
-    fill_sectors(hz, N, -static_cast<double>(c) - 2e-3, static_cast<double>(c) + 2e-3, true,
-                 sectors);
+    const bool decide = a.sun_count >= kDecideMinTimes;
+    if (decide)
+      fill_sectors(hz, N, -static_cast<double>(c) - 2e-3, static_cast<double>(c) + 2e-3, true,
+                   sectors);
     for (int t = 0; t < a.sun_count; ++t) {
       double e[3], horizontal;
       body_vector(f, a.sun + 3 * t, e, horizontal);
-      const int j = sector_of(e[0], e[1]) * sectors.stride;
+      const int j = decide ? sector_of(e[0], e[1]) * sectors.stride : 0;
       float fr;
-      if (e[2] < horizontal * static_cast<double>(sectors.dark[j])) {
+      if (decide && e[2] < horizontal * static_cast<double>(sectors.dark[j])) {
         fr = 0.0f;
-      } else if (e[2] > horizontal * static_cast<double>(sectors.lit[j])) {
+      } else if (decide && e[2] > horizontal * static_cast<double>(sectors.lit[j])) {
         fr = 1.0f;
       } else {
         double az, el;
@@ -399,7 +404,7 @@ LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origi
     int visible = 0;
     // the visible count alone is decided for the whole row where the Earth's
     // centre clears the row's extremes; the margins need every value
-    const bool decide = a.earth_margin == nullptr;
+    const bool decide = a.earth_margin == nullptr && a.earth_count >= kDecideMinTimes;
     if (decide) fill_sectors(hz, N, -1e-4, 1e-4, false, sectors);
     for (int t = 0; t < a.earth_count; ++t) {
       double e[3], horizontal;
@@ -429,8 +434,9 @@ LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origi
     const bool mask_only = a.psr_margin == nullptr;
     // with the mask alone, a vector below the row's lowest bin (less the
     // threshold) cannot light the limb and one above its highest does
-    const RowDecision rd = mask_only ? margin_decision(row_extremes(hz, N), a.psr_threshold)
-                                     : RowDecision{-INFINITY, INFINITY};
+    const RowDecision rd = mask_only && a.psr_count >= kDecideMinTimes
+                               ? margin_decision(row_extremes(hz, N), a.psr_threshold)
+                               : RowDecision{-INFINITY, INFINITY};
     double best = -1e300;
     for (int t = 0; t < a.psr_count; ++t) {
       double e[3], horizontal;

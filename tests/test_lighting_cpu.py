@@ -233,8 +233,10 @@ def test_row_decisions_leave_fractions_visibility_and_mask_unchanged():
     rng = np.random.default_rng(2)
     az = np.radians(np.arange(1440) * 0.25)
     hz = (rng.uniform(-2.5, 4.0, (128, 128, 1)) + 1.2 * np.sin(3 * az)[None, None, :]).astype(np.float32)
-    sun = synthetic.sun_vectors(dem, 96, 4.0, seed=8)
-    earth = synthetic.earth_vectors(96, 4.0, seed=9)
+    hz[:24] += np.float32(8.0)  # pixels that no Sun vector reaches (PSR)
+    # series of at least 512 times (shorter ones take no decisions)
+    sun = synthetic.sun_vectors(dem, 600, 4.0, seed=8)
+    earth = synthetic.earth_vectors(600, 4.0, seed=9)
     reduced, _ = oracle.psr_reduce(dem, sun)
     rows = hz.reshape(1, 16384, 1440)
     common = dict(sun=sun, earth=earth, psr_vectors=reduced)
@@ -270,6 +272,7 @@ def test_row_decisions_see_the_walk_extrapolate_left_of_bin_zero():
         hz = np.broadcast_to(rowv, (128, 128, 1440)).astype(np.float32)
         azs = [0.02, 0.1, 0.2, 0.3, 0.38, 0.5, 359.9]
         vec = np.stack([local_direction_vector(dem, row, col, a, el, 1.496e11) for a in azs])
+        vec = np.tile(vec, (80, 1))  # 560 times: long enough to take decisions
         out = CpuLightingSession(threads=2).lighting_tiles(
             dem, [(0, 128)], hz.reshape(1, 16384, 1440), sun=vec, sun_fraction=True)
         got = out["sun_fraction"][0].reshape(-1, 128, 128)
