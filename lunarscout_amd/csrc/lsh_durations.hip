@@ -18,10 +18,13 @@ namespace {
 
 template <bool kAzimuthMajor>
 __global__ void __launch_bounds__(256) duration_kernel(DurationArgs a) {
+  // each thread's sector tables of the sun signal, [sector][thread]
+  __shared__ float s_dark[kSectors][256], s_lit[kSectors][256];
   const int tile = blockIdx.y;
   const int pixel = blockIdx.x * blockDim.x + threadIdx.x;
   if (pixel >= kPatchPixels) return;
-  duration_pixel<kAzimuthMajor>(a, tile, pixel, a.light.tiles[tile]);
+  const SectorTable sectors{&s_dark[0][threadIdx.x], &s_lit[0][threadIdx.x], 256};
+  duration_pixel<kAzimuthMajor>(a, tile, pixel, a.light.tiles[tile], sectors);
 }
 
 }  // namespace
@@ -41,12 +44,14 @@ void run_durations_host(const DurationArgs& a, int threads) {
   n = n < 1 ? 1 : (n > 256 ? 256 : n);
   auto body = [&a, items, n](int t) {
     const int64_t begin = items * t / n, end = items * (t + 1) / n;
+    float dark[kSectors], lit[kSectors];
+    const SectorTable sectors{dark, lit, 1};
     for (int64_t i = begin; i < end; ++i) {
       const int tile = static_cast<int>(i / kPatchPixels), pixel = static_cast<int>(i % kPatchPixels);
       if (a.light.azimuth_major)
-        duration_pixel<true>(a, tile, pixel, a.light.tiles[tile]);
+        duration_pixel<true>(a, tile, pixel, a.light.tiles[tile], sectors);
       else
-        duration_pixel<false>(a, tile, pixel, a.light.tiles[tile]);
+        duration_pixel<false>(a, tile, pixel, a.light.tiles[tile], sectors);
     }
   };
   std::vector<std::thread> pool;

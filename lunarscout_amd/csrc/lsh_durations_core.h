@@ -59,14 +59,21 @@ struct PixelSignals {
   const LightArgs& a;
   Frame f;
   HorizonRow<kAzimuthMajor> hz;
-  RowDecision rd;  // whole-row decisions of the sun fraction (row_decision)
+  // per-sector decisions of the sun signal (fill_sectors): for the fraction the
+  // disk bounds (0 below, 1 above); for the centre margin the bounds of its
+  // sign against the threshold (-inf below, +inf above: the reducers only
+  // compare the signal with the threshold). Off for short series.
+  SectorTable sectors;
+  bool decide;
   // sun fraction (BuilderSunFraction) or centre margin el - lerp(horizon), float32
   LSH_LHD float sun(int t, int signal) const {
     double e[3], horizontal;
     body_vector(f, a.sun + 3 * t, e, horizontal);
-    if (signal == LSH_SIGNAL_SUN_FRACTION) {
-      if (e[2] < horizontal * rd.tan_dark) return 0.0f;
-      if (e[2] > horizontal * rd.tan_lit) return 1.0f;
+    if (decide) {
+      const int j = sector_of(e[0], e[1]) * sectors.stride;
+      const bool fraction = signal == LSH_SIGNAL_SUN_FRACTION;
+      if (e[2] < horizontal * static_cast<double>(sectors.dark[j])) return fraction ? 0.0f : -INFINITY;
+      if (e[2] > horizontal * static_cast<double>(sectors.lit[j])) return fraction ? 1.0f : INFINITY;
     }
     double az, el;
     azimuth_elevation_of(e, horizontal, az, el);
@@ -122,7 +129,8 @@ LSH_LHD void close_low_sun_run(const DurationArgs& a, int64_t slot, int64_t stri
 }
 
 template <bool kAzimuthMajor>
-LSH_LHD void duration_pixel(const DurationArgs& a, int tile, int pixel, lsh_tile origin) {
+LSH_LHD void duration_pixel(const DurationArgs& a, int tile, int pixel, lsh_tile origin,
+                            SectorTable sectors) {
   const LightArgs& la = a.light;
   const int line = pixel / 128, sample = pixel % 128;
   const int row = origin.tile_row + line, column = origin.tile_column + sample;
@@ -133,12 +141,23 @@ LSH_LHD void duration_pixel(const DurationArgs& a, int tile, int pixel, lsh_tile
     for (int k = 0; k < bands; ++k) a.out[k * stride + slot] = NAN;
     return;
   }
-  PixelSignals<kAzimuthMajor> sig{la, pixel_frame(la, row, column), {}, {-INFINITY, INFINITY}};
+  PixelSignals<kAzimuthMajor> sig{la, pixel_frame(la, row, column), {}, sectors, false};
   sig.hz.N = la.N;
   sig.hz.base = kAzimuthMajor ? la.horizons + static_cast<int64_t>(tile) * kPatchPixels * la.N + pixel
                               : la.horizons + slot * la.N;
-  if (a.sun_signal == LSH_SIGNAL_SUN_FRACTION && a.T >= kDecideMinTimes)
-    sig.rd = row_decision(sig.hz, la.N, la);
+  // the safe-haven reducer reads the fraction whatever sun_signal says
+  const int signal =
+      a.reducer == LSH_REDUCE_SAFE_HAVEN ? LSH_SIGNAL_SUN_FRACTION : a.sun_signal;
+  sig.decide = a.T >= kDecideMinTimes;
+  if (sig.decide && signal == LSH_SIGNAL_SUN_FRACTION) {
+    float c = 0.0f;  // the tallest slice half-height
+    for (int i = 0; i < 16; ++i) c = la.half_circle[i] > c ? la.half_circle[i] : c;
+    fill_sectors(sig.hz, la.N, -static_cast<double>(c) - 2e-3, static_cast<double>(c) + 2e-3,
+                 true, sectors);
+  } else if (sig.decide) {
+    fill_sectors(sig.hz, la.N, static_cast<double>(a.sun_threshold) - 1e-4,
+                 static_cast<double>(a.sun_threshold) + 1e-4, false, sectors);
+  }
 
   if (a.reducer == LSH_REDUCE_LONGEST_DURATION) {
     for (int k = 0; k < a.K; ++k) a.longest[k * stride + slot] = 0.0;
