@@ -96,15 +96,17 @@ LSH_LHD void azimuth_elevation(const Frame& f, const double* v, double& az, doub
   azimuth_elevation_of(e, horizontal, az, el);
 }
 
-// Whole-row decisions of the sun fraction. With lo and hi the lowest and
-// highest bins of a row (every bin finite) and c the tallest slice half-height,
-// a disk whose centre elevation is below lo - c - 2e-3 deg has every slice top
-// below every interpolated horizon of the row, so the walk adds nothing (0); one
-// above hi + c + 2e-3 deg has every slice wholly above, so the walk adds every
-// full slice (max_photons exactly, 1). The margin is far above the float32
-// rounding of the walk. Both tests are taken on the body vector before any
-// arctangent: el < t <=> u < |(e, n)| tan(t) for |t| < 90 deg. A row with a
-// non-finite bin, or a bound beyond +-89 deg, decides nothing.
+// Decisions from horizon extremes. With lo and hi the lowest and highest of the
+// bins a body's test can read (every one finite) and c the tallest slice
+// half-height, a disk whose centre elevation is below lo - c - 2e-3 deg has
+// every slice top below every interpolated horizon, so the walk adds nothing
+// (0); one above hi + c + 2e-3 deg has every slice wholly above, so the walk
+// adds every full slice (max_photons exactly, 1). The margin is far above the
+// float32 rounding of the walk. Likewise a margin el - lerp(horizon) lies below
+// a threshold when el < lo + threshold - 1e-4 deg and above it when
+// el > hi + threshold + 1e-4 deg. Every test is taken on the body vector before
+// any arctangent: el < t <=> u < |(e, n)| tan(t) for |t| < 90 deg. Bins that are
+// not all finite, or a bound beyond +-89 deg, decide nothing.
 struct RowExtremes {
   float lo, hi;  // lowest and highest bin
   bool finite;   // every bin finite
@@ -140,30 +142,6 @@ LSH_LHD void add_walk_extrapolation(const Row& hz, int N, RowExtremes& x) {
 LSH_LHD double tan_bound(bool usable, double deg, double none) {
   return usable && deg > -89.0 && deg < 89.0 ? tan(deg * (kPi / 180.0)) : none;
 }
-struct RowDecision {
-  double tan_dark, tan_lit;
-};
-LSH_LHD RowDecision row_decision(const RowExtremes& x, const LightArgs& a) {
-  float c = 0.0f;
-  for (int i = 0; i < 16; ++i) c = a.half_circle[i] > c ? a.half_circle[i] : c;
-  return RowDecision{tan_bound(x.finite, static_cast<double>(x.lo) - c - 2e-3, -INFINITY),
-                     tan_bound(x.finite, static_cast<double>(x.hi) + c + 2e-3, INFINITY)};
-}
-template <class Row>
-LSH_LHD RowDecision row_decision(const Row& hz, int N, const LightArgs& a) {
-  RowExtremes x = row_extremes(hz, N);
-  add_walk_extrapolation(hz, N, x);
-  return row_decision(x, a);
-}
-// The same for the sign of a margin el - lerp(horizon) against a threshold:
-// below lo + threshold - 1e-6 deg it is below the threshold wherever the body
-// stands, above hi + threshold + 1e-6 deg above it (the margin's float64
-// rounding is ~1e-13 deg).
-LSH_LHD RowDecision margin_decision(const RowExtremes& x, double threshold) {
-  return RowDecision{tan_bound(x.finite, static_cast<double>(x.lo) + threshold - 1e-6, -INFINITY),
-                     tan_bound(x.finite, static_cast<double>(x.hi) + threshold + 1e-6, INFINITY)};
-}
-
 // Sector decisions: the same tests on the extremes of the bins of one of 16
 // azimuth sectors of 22.5 deg, widened by 4 bins (1 deg) on either side -- more
 // than the disk walk reaches beyond the centre's azimuth (-0.4 .. +0.86 deg) or
@@ -434,17 +412,19 @@ LSH_LHD void light_pixel(const LightArgs& a, int tile, int pixel, lsh_tile origi
     const bool mask_only = a.psr_margin == nullptr;
     // with the mask alone, a vector below the row's lowest bin (less the
     // threshold) cannot light the limb and one above its highest does
-    const RowDecision rd = mask_only && a.psr_count >= kDecideMinTimes
-                               ? margin_decision(row_extremes(hz, N), a.psr_threshold)
-                               : RowDecision{-INFINITY, INFINITY};
+    const bool decide = mask_only && a.psr_count >= kDecideMinTimes;
+    if (decide) fill_sectors(hz, N, a.psr_threshold - 1e-4, a.psr_threshold + 1e-4, false, sectors);
     double best = -1e300;
     for (int t = 0; t < a.psr_count; ++t) {
       double e[3], horizontal;
       body_vector(f, a.psr_vectors + 3 * t, e, horizontal);
-      if (e[2] < horizontal * rd.tan_dark) continue;
-      if (e[2] > horizontal * rd.tan_lit) {
-        best = INFINITY;
-        break;
+      if (decide) {
+        const int j = sector_of(e[0], e[1]) * sectors.stride;
+        if (e[2] < horizontal * static_cast<double>(sectors.dark[j])) continue;
+        if (e[2] > horizontal * static_cast<double>(sectors.lit[j])) {
+          best = INFINITY;
+          break;
+        }
       }
       double az, el;
       azimuth_elevation_of(e, horizontal, az, el);
