@@ -18,13 +18,16 @@ namespace {
 
 template <bool kAzimuthMajor>
 __global__ void __launch_bounds__(256) duration_kernel(DurationArgs a) {
-  // each thread's sector tables of the sun signal, [sector][thread]
+  // each thread's sector tables of the sun signal and the Earth margin,
+  // [sector][thread]
   __shared__ float s_dark[kSectors][256], s_lit[kSectors][256];
+  __shared__ float s_earth_dark[kSectors / 2][256], s_earth_lit[kSectors / 2][256];
   const int tile = blockIdx.y;
   const int pixel = blockIdx.x * blockDim.x + threadIdx.x;
   if (pixel >= kPatchPixels) return;
   const SectorTable sectors{&s_dark[0][threadIdx.x], &s_lit[0][threadIdx.x], 256};
-  duration_pixel<kAzimuthMajor>(a, tile, pixel, a.light.tiles[tile], sectors);
+  const SectorTable earth_sectors{&s_earth_dark[0][threadIdx.x], &s_earth_lit[0][threadIdx.x], 256};
+  duration_pixel<kAzimuthMajor>(a, tile, pixel, a.light.tiles[tile], sectors, earth_sectors);
 }
 
 }  // namespace
@@ -44,14 +47,14 @@ void run_durations_host(const DurationArgs& a, int threads) {
   n = n < 1 ? 1 : (n > 256 ? 256 : n);
   auto body = [&a, items, n](int t) {
     const int64_t begin = items * t / n, end = items * (t + 1) / n;
-    float dark[kSectors], lit[kSectors];
-    const SectorTable sectors{dark, lit, 1};
+    float dark[kSectors], lit[kSectors], earth_dark[kSectors / 2], earth_lit[kSectors / 2];
+    const SectorTable sectors{dark, lit, 1}, earth_sectors{earth_dark, earth_lit, 1};
     for (int64_t i = begin; i < end; ++i) {
       const int tile = static_cast<int>(i / kPatchPixels), pixel = static_cast<int>(i % kPatchPixels);
       if (a.light.azimuth_major)
-        duration_pixel<true>(a, tile, pixel, a.light.tiles[tile], sectors);
+        duration_pixel<true>(a, tile, pixel, a.light.tiles[tile], sectors, earth_sectors);
       else
-        duration_pixel<false>(a, tile, pixel, a.light.tiles[tile], sectors);
+        duration_pixel<false>(a, tile, pixel, a.light.tiles[tile], sectors, earth_sectors);
     }
   };
   std::vector<std::thread> pool;

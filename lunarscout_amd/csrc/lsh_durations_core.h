@@ -65,6 +65,11 @@ struct PixelSignals {
   // compare the signal with the threshold). Off for short series.
   SectorTable sectors;
   bool decide;
+  // per-sector decisions of the Earth margin's sign against its threshold (8
+  // sectors of 45 deg: the workgroup's LDS holds both tables without costing
+  // the kernel a resident workgroup)
+  SectorTable earth_sectors;
+  bool decide_earth;
   // sun fraction (BuilderSunFraction) or centre margin el - lerp(horizon), float32
   LSH_LHD float sun(int t, int signal) const {
     double e[3], horizontal;
@@ -82,8 +87,15 @@ struct PixelSignals {
     return static_cast<float>(el - horizon_at(hz, a.N, az));
   }
   LSH_LHD float earth(int t) const {
+    double e[3], horizontal;
+    body_vector(f, a.earth + 3 * t, e, horizontal);
+    if (decide_earth) {
+      const int j = (sector_of(e[0], e[1]) >> 1) * earth_sectors.stride;
+      if (e[2] < horizontal * static_cast<double>(earth_sectors.dark[j])) return -INFINITY;
+      if (e[2] > horizontal * static_cast<double>(earth_sectors.lit[j])) return INFINITY;
+    }
     double az, el;
-    azimuth_elevation(f, a.earth + 3 * t, az, el);
+    azimuth_elevation_of(e, horizontal, az, el);
     return static_cast<float>(el - horizon_at(hz, a.N, az));
   }
 };
@@ -130,7 +142,7 @@ LSH_LHD void close_low_sun_run(const DurationArgs& a, int64_t slot, int64_t stri
 
 template <bool kAzimuthMajor>
 LSH_LHD void duration_pixel(const DurationArgs& a, int tile, int pixel, lsh_tile origin,
-                            SectorTable sectors) {
+                            SectorTable sectors, SectorTable earth_sectors) {
   const LightArgs& la = a.light;
   const int line = pixel / 128, sample = pixel % 128;
   const int row = origin.tile_row + line, column = origin.tile_column + sample;
@@ -141,7 +153,8 @@ LSH_LHD void duration_pixel(const DurationArgs& a, int tile, int pixel, lsh_tile
     for (int k = 0; k < bands; ++k) a.out[k * stride + slot] = NAN;
     return;
   }
-  PixelSignals<kAzimuthMajor> sig{la, pixel_frame(la, row, column), {}, sectors, false};
+  PixelSignals<kAzimuthMajor> sig{la, pixel_frame(la, row, column), {}, sectors, false,
+                                  earth_sectors, false};
   sig.hz.N = la.N;
   sig.hz.base = kAzimuthMajor ? la.horizons + static_cast<int64_t>(tile) * kPatchPixels * la.N + pixel
                               : la.horizons + slot * la.N;
@@ -158,6 +171,13 @@ LSH_LHD void duration_pixel(const DurationArgs& a, int tile, int pixel, lsh_tile
     fill_sectors(sig.hz, la.N, static_cast<double>(a.sun_threshold) - 1e-4,
                  static_cast<double>(a.sun_threshold) + 1e-4, false, sectors);
   }
+  // the safe haven reads the Earth margin at every time (the longest durations
+  // only where the Sun holds, too seldom to repay the table)
+  sig.decide_earth = sig.decide && a.reducer == LSH_REDUCE_SAFE_HAVEN;
+  if (sig.decide_earth)
+    fill_sectors<kSectors / 2>(sig.hz, la.N, static_cast<double>(a.earth_threshold) - 1e-4,
+                               static_cast<double>(a.earth_threshold) + 1e-4, false,
+                               earth_sectors);
 
   if (a.reducer == LSH_REDUCE_LONGEST_DURATION) {
     for (int k = 0; k < a.K; ++k) a.longest[k * stride + slot] = 0.0;

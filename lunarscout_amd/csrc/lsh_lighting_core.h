@@ -171,7 +171,8 @@ LSH_LHD int sector_of(double east, double north) {
 }
 // below: deg offset of the "below" bound from a sector's lowest bin (negative),
 // above: of the "above" bound from its highest bin
-template <class Row>
+// (kS sectors of 360 / kS deg: 16, or 8 -- pairs of the 16, sector_of / 2)
+template <int kS = kSectors, class Row>
 LSH_LHD void fill_sectors(const Row& hz, int N, double below, double above, bool disk_walk,
                           SectorTable t) {
   if (N != 1440) {
@@ -179,14 +180,14 @@ LSH_LHD void fill_sectors(const Row& hz, int N, double below, double above, bool
     if (disk_walk) add_walk_extrapolation(hz, N, x);
     const float d = static_cast<float>(tan_bound(x.finite, static_cast<double>(x.lo) + below, -INFINITY));
     const float l = static_cast<float>(tan_bound(x.finite, static_cast<double>(x.hi) + above, INFINITY));
-    for (int j = 0; j < kSectors; ++j) {
+    for (int j = 0; j < kS; ++j) {
       t.dark[j * t.stride] = d;
       t.lit[j * t.stride] = l;
     }
     return;
   }
-  constexpr int kWidth = 1440 / kSectors, kExtra = 4;
-  for (int j = 0; j < kSectors; ++j) {
+  constexpr int kWidth = 1440 / kS, kExtra = 4;
+  for (int j = 0; j < kS; ++j) {
     RowExtremes x{1e30f, -1e30f, true};
     for (int k = -kExtra; k < kWidth + kExtra; ++k) {
       int b = j * kWidth + k;
@@ -197,7 +198,7 @@ LSH_LHD void fill_sectors(const Row& hz, int N, double below, double above, bool
       x.hi = v > x.hi ? v : x.hi;
     }
     // the sectors on either side of azimuth 0 (a centre there may fall in either)
-    if (disk_walk && (j == 0 || j == kSectors - 1)) add_walk_extrapolation(hz, N, x);
+    if (disk_walk && (j == 0 || j == kS - 1)) add_walk_extrapolation(hz, N, x);
     t.dark[j * t.stride] =
         static_cast<float>(tan_bound(x.finite, static_cast<double>(x.lo) + below, -INFINITY));
     t.lit[j * t.stride] =
